@@ -1,0 +1,117 @@
+/* Extended C ABI of the MI355X batched small-volume 3D FFT.
+ *
+ * The reference's ABIs (fft3d_api.h, fft3d_gpu_api.h) are forward-only, cubes-only and
+ * complex128-only (bench/geom/fft3d_api.h:5-11,57-61).  This header adds, beside them:
+ * anisotropic extents, the backward direction, complex64, explicit streams, a per-call
+ * batch, kernel enumeration and forcing, and an error channel.  Conventions follow the
+ * reference's definition module: forward exponent -1, backward +1, unnormalised unless
+ * FFT3D_MX_NORMALIZE asks for 1/V (python/slow_dft.py:11-18,27-28,109-113).
+ *
+ * Layout for every entry point: B contiguous volumes, element (b,x,y,z) at
+ * ((b*n0 + x)*n1 + y)*n2 + z, interleaved complex (double {re,im} or float {re,im}).
+ * Pointers passed to execute are DEVICE pointers; `in` is never modified.
+ *
+ * Error convention: functions returning int give 0 on success and a negative
+ * FFT3D_MX_E* code on failure; functions returning a pointer give NULL.  The message of
+ * the most recent failure on the calling thread is fft3d_mx_last_error().
+ */
+#ifndef FFT3D_MI355X_H
+#define FFT3D_MI355X_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FFT3D_MX_FORWARD = -1,
+    FFT3D_MX_BACKWARD = +1
+};
+enum {
+    FFT3D_MX_C128 = 0, /* complex double, 16 bytes per element */
+    FFT3D_MX_C64 = 1   /* complex float, 8 bytes per element */
+};
+/* create() flags */
+enum {
+    FFT3D_MX_NORMALIZE = 1u, /* multiply the result by 1/(n0*n1*n2) */
+    FFT3D_MX_NO_TUNE = 2u    /* skip the create-time race; take the first registered kernel */
+};
+/* error codes */
+enum {
+    FFT3D_MX_OK = 0,
+    FFT3D_MX_EINVAL = -1,      /* bad argument */
+    FFT3D_MX_EUNSUPPORTED = -2,/* no kernel for this geometry / precision / direction */
+    FFT3D_MX_EHIP = -3,        /* a HIP runtime call failed */
+    FFT3D_MX_ENOMEM = -4       /* device or host allocation failed */
+};
+
+typedef struct fft3d_mx_plan fft3d_mx_plan;
+
+/* Message of the last failure on this thread ("" if none).  Never NULL. */
+const char *fft3d_mx_last_error(void);
+/* Clear the last-failure message. */
+void fft3d_mx_clear_error(void);
+
+/* Nonzero if a kernel exists for (n0, n1, n2) at this precision (both directions exist). */
+int fft3d_mx_supports(int n0, int n1, int n2, int precision);
+
+/* Plan a transform of up to `batch` volumes on the current HIP device.  Computes the
+ * twiddle / Rader tables in long double, uploads them, and -- when several kernels can
+ * serve the geometry -- times each on the device and keeps the fastest (the choice is
+ * cached per process, and in the file named by $FFT3D_MX_WISDOM when that is set).
+ * $FFT3D_MX_KERNEL=<name> forces a kernel by name.  NULL on failure. */
+fft3d_mx_plan *fft3d_mx_create(int n0, int n1, int n2, int batch, int direction,
+                               int precision, unsigned flags);
+
+/* Same, but with the kernel chosen by name (see fft3d_mx_kernel_name); no race. */
+fft3d_mx_plan *fft3d_mx_create_named(int n0, int n1, int n2, int batch, int direction,
+                                     int precision, unsigned flags, const char *kernel);
+
+/* Enqueue the transform of the plan's batch on `stream` (a hipStream_t; NULL = the NULL
+ * stream).  Asynchronous.  0 or a negative error code. */
+int fft3d_mx_execute(fft3d_mx_plan *plan, const void *in, void *out, void *stream);
+
+/* Same for the first `batch` volumes only (0 <= batch <= the planned batch). */
+int fft3d_mx_execute_n(fft3d_mx_plan *plan, const void *in, void *out, long batch,
+                       void *stream);
+
+void fft3d_mx_destroy(fft3d_mx_plan *plan);
+
+/* Introspection of a live plan. */
+typedef struct fft3d_mx_plan_info {
+    int n0, n1, n2, batch, direction, precision;
+    int kind;            /* 0 fused (volume resident in LDS), 1 pencil (two passes) */
+    int volumes_per_wg;  /* fused: volumes per workgroup; pencil: slabs per workgroup */
+    int threads;         /* workgroup size of the main kernel */
+    int variant;         /* line-algorithm variant (L=17: 0 Rader, 1 folded dense) */
+    size_t lds_bytes;    /* LDS per workgroup of the main kernel */
+    double tune_us;      /* create-time race result of the winner (0 if no race) */
+    char kernel[96];     /* kernel name */
+} fft3d_mx_plan_info;
+int fft3d_mx_plan_get_info(const fft3d_mx_plan *plan, fft3d_mx_plan_info *info);
+
+/* Kernel registry enumeration: every compiled kernel variant. */
+int fft3d_mx_kernel_count(void);
+/* Name of kernel i and its geometry/precision/direction; NULL if i is out of range. */
+const char *fft3d_mx_kernel_name(int i, int *n0, int *n1, int *n2, int *precision,
+                                 int *direction);
+
+/* The plan-time constant bank, exactly as uploaded to the device, for inspection.  Layout
+ * (element type double for FFT3D_MX_C128, float for FFT3D_MX_C64, complex = {re, im}):
+ *   W tables, complex, concatenated for N = 2,3,4,6,8,9,12,16,17,24,36,48 (185 entries),
+ *     entry m of table N = exp(-2 pi i m / N);
+ *   Rader-17 spectra, complex [2][16]: [0] forward, [1] backward, Bhat = DFT16(b')/16,
+ *     b'[q] = W17^(3^-q mod 17) with W17 = exp(direction * 2 pi i / 17);
+ *   cos(2 pi m/17), m = 0..16, real [17];  sin(2 pi m/17), m = 0..16, real [17].
+ * Writes at most `cap` bytes to dst (dst may be NULL to query *size).  0 or error. */
+int fft3d_mx_bank_image(int precision, void *dst, size_t cap, size_t *size);
+
+/* Bytes of device memory one volume of this geometry/precision occupies. */
+size_t fft3d_mx_volume_bytes(int n0, int n1, int n2, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FFT3D_MI355X_H */

@@ -1,0 +1,311 @@
+// The two 3D kernel shapes, as templates over geometry, precision, direction and tiling.
+//
+// Fused (volume fits in LDS; L = 6, 8, 12, 17, 8x8x16):
+//   one workgroup owns VPW consecutive volumes.  One coalesced HBM read stages them into
+//   LDS (z rows padded by PZ elements, x planes by PY, to keep the strided line reads off a
+//   single bank), the z and y line passes run LDS -> VGPR -> LDS, and the x pass reads LDS
+//   and writes HBM directly: lanes of a wave take consecutive (y,z), so every store of the
+//   x pass is one contiguous, coalesced run.  One HBM read and one HBM write per element.
+//
+// Pencil (volume exceeds LDS; L = 24, 36, 24x24x48 at complex128):
+//   pass A: one workgroup owns SPW consecutive x-slabs (contiguous n1*n2 planes), stages
+//   them in LDS, runs z then y; the y pass writes the slab straight to `out`, coalesced.
+//   pass B: one thread per x-line, in place on `out`: every load and store instruction of a
+//   wave covers consecutive (y,z), i.e. contiguous addresses, so no LDS is needed at all.
+//   Pass B re-reads what pass A wrote; chunking keeps that intermediate cache resident.
+//
+// Each kernel body is a sequence of phase functions separated by workgroup barriers.  A
+// phase only reads LDS locations written in an earlier phase or by the same thread, so the
+// phases are plain __host__ __device__ functions of (thread, block).  Launchers come from
+// <fx_launch.hpp>, found on the include path.
+#pragma once
+#include "codelets.hpp"
+#include "registry.hpp"
+
+namespace fx {
+
+template <typename T>
+constexpr int prec_of() { return sizeof(T) == 8 ? kC128 : kC64; }
+
+// Copy TOT contiguous elements HBM -> LDS, slot(e) giving the padded LDS index.  Loads
+// are issued in groups of up to 16 per thread before any LDS write, so each wave keeps
+// 16 x 1 KiB (complex128) of reads in flight.
+template <int TOT, int NT, typename T, typename SlotFn>
+FX_HD void stage_full(int tid, const cx<T>* __restrict__ src, cx<T>* sm, SlotFn slot)
+{
+    constexpr int ITER = (TOT + NT - 1) / NT;
+    constexpr int GROUP = ITER < 16 ? ITER : 16;
+#pragma unroll
+    for (int g = 0; g < ITER; g += GROUP) {
+        cx<T> buf[GROUP];
+        // Loads are unconditional (a tail lane re-reads the last element) so the compiler
+        // keeps buf in VGPRs and issues the whole group before the first wait.
+#pragma unroll
+        for (int k = 0; k < GROUP; ++k) {
+            const int e = tid + (g + k) * NT;
+            if (g + k < ITER) buf[k] = src[(TOT % NT == 0 || e < TOT) ? e : TOT - 1];
+        }
+#pragma unroll
+        for (int k = 0; k < GROUP; ++k) {
+            const int e = tid + (g + k) * NT;
+            if (g + k < ITER && (TOT % NT == 0 || e < TOT)) sm[slot(e)] = buf[k];
+        }
+    }
+}
+
+// In-LDS line pass: line l starts at base(l), its N points are STRIDE apart.
+template <int N, int STRIDE, int S, int V17, int NT, typename T, typename BaseFn>
+FX_HD void lds_pass(int tid, cx<T>* sm, int nlines, BaseFn base, const Bank<T>* __restrict__ bk)
+{
+    for (int l = tid; l < nlines; l += NT) {
+        const int b = base(l);
+        cx<T> t[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) t[j] = sm[b + j * STRIDE];
+        dft<N, S, T, V17>(t, bk);
+#pragma unroll
+        for (int j = 0; j < N; ++j) sm[b + j * STRIDE] = t[j];
+    }
+}
+
+// ------------------------------------------------------------------------------- fused
+template <int N0_, int N1_, int N2_, typename T_, int S_, int VPW_, int NT_, int PZ_, int PY_,
+          int V17_>
+struct FusedCfg {
+    using T = T_;
+    static constexpr int N0 = N0_, N1 = N1_, N2 = N2_, S = S_, VPW = VPW_, NT = NT_;
+    static constexpr int PZ = PZ_, PY = PY_, V17 = V17_;
+    static constexpr int V = N0 * N1 * N2;
+    static constexpr int PLANE = N1 * N2;
+    static constexpr int YS = N2 + PZ;           // LDS pitch between z rows
+    static constexpr int XS = N1 * YS + PY;      // LDS pitch between x planes
+    static constexpr int VS = N0 * XS;           // LDS pitch between volumes
+    static constexpr int LDS = VPW * VS;         // elements
+};
+
+template <class C>
+struct Fused {
+    using T = typename C::T;
+
+    // Volumes held by block `bid` (VPW except for the last block).
+    static FX_HD int nvols(long bid, long nvol)
+    {
+        const long left = nvol - bid * C::VPW;
+        return (int)(left < C::VPW ? left : C::VPW);
+    }
+    static FX_HD int slot(int e)
+    {
+        const int v = e / C::V;
+        const int r = e - v * C::V;
+        const int x = r / C::PLANE;
+        const int q = r - x * C::PLANE;
+        const int y = q / C::N2;
+        const int z = q - y * C::N2;
+        return v * C::VS + x * C::XS + y * C::YS + z;
+    }
+    // Phase 1: one coalesced read of the block's contiguous volumes into LDS.
+    static FX_HD void stage(int tid, long bid, long nvol, const cx<T>* __restrict__ in, cx<T>* sm)
+    {
+        const int nv = nvols(bid, nvol);
+        const cx<T>* __restrict__ src = in + bid * C::VPW * (long)C::V;
+        if (nv == C::VPW)
+            stage_full<C::VPW * C::V, C::NT>(tid, src, sm, [](int e) { return slot(e); });
+        else
+            for (int e = tid; e < nv * C::V; e += C::NT) sm[slot(e)] = src[e];
+    }
+    // Phase 2: z lines (v, x, y), contiguous in LDS.
+    static FX_HD void zpass(int tid, long bid, long nvol, cx<T>* sm, const Bank<T>* __restrict__ bk)
+    {
+        lds_pass<C::N2, 1, C::S, C::V17, C::NT>(
+            tid, sm, nvols(bid, nvol) * C::N0 * C::N1,
+            [](int l) {
+                const int v = l / (C::N0 * C::N1);
+                const int r = l - v * (C::N0 * C::N1);
+                const int x = r / C::N1;
+                const int y = r - x * C::N1;
+                return v * C::VS + x * C::XS + y * C::YS;
+            },
+            bk);
+    }
+    // Phase 3: y lines (v, x, z), stride YS.
+    static FX_HD void ypass(int tid, long bid, long nvol, cx<T>* sm, const Bank<T>* __restrict__ bk)
+    {
+        lds_pass<C::N1, C::YS, C::S, C::V17, C::NT>(
+            tid, sm, nvols(bid, nvol) * C::N0 * C::N2,
+            [](int l) {
+                const int v = l / (C::N0 * C::N2);
+                const int r = l - v * (C::N0 * C::N2);
+                const int x = r / C::N2;
+                const int z = r - x * C::N2;
+                return v * C::VS + x * C::XS + z;
+            },
+            bk);
+    }
+    // Phase 4: x lines (v, y, z), LDS -> HBM; a wave's stores are contiguous in (y, z).
+    static FX_HD void xpass(int tid, long bid, long nvol, const cx<T>* sm, cx<T>* __restrict__ out,
+                            const Bank<T>* __restrict__ bk, T scl)
+    {
+        const int nv = nvols(bid, nvol);
+        cx<T>* __restrict__ dst = out + bid * C::VPW * (long)C::V;
+        const bool scaled = scl != T(1);
+        for (int l = tid; l < nv * C::PLANE; l += C::NT) {
+            const int v = l / C::PLANE;
+            const int q = l - v * C::PLANE;
+            const int y = q / C::N2;
+            const int z = q - y * C::N2;
+            const int b = v * C::VS + y * C::YS + z;
+            cx<T> t[C::N0];
+#pragma unroll
+            for (int j = 0; j < C::N0; ++j) t[j] = sm[b + j * C::XS];
+            dft<C::N0, C::S, T, C::V17>(t, bk);
+            cx<T>* __restrict__ o = dst + (long)v * C::V + q;
+            if (scaled) {
+#pragma unroll
+                for (int k = 0; k < C::N0; ++k) o[k * C::PLANE] = scale(t[k], scl);
+            } else {
+#pragma unroll
+                for (int k = 0; k < C::N0; ++k) o[k * C::PLANE] = t[k];
+            }
+        }
+    }
+};
+
+template <class C>
+__global__ void __launch_bounds__(C::NT)
+    k_fused(const cx<typename C::T>* __restrict__ in, cx<typename C::T>* __restrict__ out,
+            long nvol, const Bank<typename C::T>* __restrict__ bk, typename C::T scl)
+{
+    using K = Fused<C>;
+    __shared__ cx<typename C::T> sm[C::LDS];
+    const int tid = threadIdx.x;
+    const long bid = blockIdx.x;
+    K::stage(tid, bid, nvol, in, sm);
+    __syncthreads();
+    K::zpass(tid, bid, nvol, sm, bk);
+    __syncthreads();
+    K::ypass(tid, bid, nvol, sm, bk);
+    __syncthreads();
+    K::xpass(tid, bid, nvol, sm, out, bk, scl);
+}
+
+// ------------------------------------------------------------------------------ pencil
+template <int N0_, int N1_, int N2_, typename T_, int S_, int SPW_, int NT_, int PZ_,
+          int NTB_, int CHUNK_>
+struct PencilCfg {
+    using T = T_;
+    static constexpr int N0 = N0_, N1 = N1_, N2 = N2_, S = S_, SPW = SPW_, NT = NT_;
+    static constexpr int PZ = PZ_, NTB = NTB_, CHUNK = CHUNK_;
+    static constexpr int V = N0 * N1 * N2;
+    static constexpr int PLANE = N1 * N2;
+    static constexpr int YS = N2 + PZ;
+    static constexpr int SS = N1 * YS;
+    static constexpr int LDS = SPW * SS;
+};
+
+template <class C>
+struct Pencil {
+    using T = typename C::T;
+
+    static FX_HD int nslabs(long bid, long nslab)
+    {
+        const long left = nslab - bid * C::SPW;
+        return (int)(left < C::SPW ? left : C::SPW);
+    }
+    static FX_HD int slot(int e)
+    {
+        const int s = e / C::PLANE;
+        const int q = e - s * C::PLANE;
+        const int y = q / C::N2;
+        const int z = q - y * C::N2;
+        return s * C::SS + y * C::YS + z;
+    }
+    // Pass A, phase 1: stage SPW contiguous (v, x) slabs.
+    static FX_HD void stage(int tid, long bid, long nslab, const cx<T>* __restrict__ in, cx<T>* sm)
+    {
+        const int ns = nslabs(bid, nslab);
+        const cx<T>* __restrict__ src = in + bid * C::SPW * (long)C::PLANE;
+        if (ns == C::SPW)
+            stage_full<C::SPW * C::PLANE, C::NT>(tid, src, sm, [](int e) { return slot(e); });
+        else
+            for (int e = tid; e < ns * C::PLANE; e += C::NT) sm[slot(e)] = src[e];
+    }
+    // Pass A, phase 2: z lines (s, y).
+    static FX_HD void zpass(int tid, long bid, long nslab, cx<T>* sm, const Bank<T>* __restrict__ bk)
+    {
+        lds_pass<C::N2, 1, C::S, 0, C::NT>(
+            tid, sm, nslabs(bid, nslab) * C::N1,
+            [](int l) {
+                const int s = l / C::N1;
+                const int y = l - s * C::N1;
+                return s * C::SS + y * C::YS;
+            },
+            bk);
+    }
+    // Pass A, phase 3: y lines (s, z), LDS -> out (coalesced across z).
+    static FX_HD void ypass(int tid, long bid, long nslab, const cx<T>* sm, cx<T>* __restrict__ out,
+                            const Bank<T>* __restrict__ bk)
+    {
+        const int ns = nslabs(bid, nslab);
+        cx<T>* __restrict__ dst = out + bid * C::SPW * (long)C::PLANE;
+        for (int l = tid; l < ns * C::N2; l += C::NT) {
+            const int s = l / C::N2;
+            const int z = l - s * C::N2;
+            const int b = s * C::SS + z;
+            cx<T> t[C::N1];
+#pragma unroll
+            for (int j = 0; j < C::N1; ++j) t[j] = sm[b + j * C::YS];
+            dft<C::N1, C::S, T>(t, bk);
+            cx<T>* __restrict__ o = dst + (long)s * C::PLANE + z;
+#pragma unroll
+            for (int k = 0; k < C::N1; ++k) o[k * C::N2] = t[k];
+        }
+    }
+    // Pass B: x-line l = (v, y, z) of buf, in place.
+    static FX_HD void xline(long l, long nlines, cx<T>* __restrict__ buf,
+                            const Bank<T>* __restrict__ bk, T scl)
+    {
+        if (l >= nlines) return;
+        const long v = l / C::PLANE;
+        const int q = (int)(l - v * C::PLANE);
+        cx<T>* __restrict__ p = buf + v * C::V + q;
+        cx<T> t[C::N0];
+#pragma unroll
+        for (int j = 0; j < C::N0; ++j) t[j] = p[j * C::PLANE];
+        dft<C::N0, C::S, T>(t, bk);
+        if (scl != T(1)) {
+#pragma unroll
+            for (int k = 0; k < C::N0; ++k) p[k * C::PLANE] = scale(t[k], scl);
+        } else {
+#pragma unroll
+            for (int k = 0; k < C::N0; ++k) p[k * C::PLANE] = t[k];
+        }
+    }
+};
+
+template <class C>
+__global__ void __launch_bounds__(C::NT)
+    k_slab(const cx<typename C::T>* __restrict__ in, cx<typename C::T>* __restrict__ out,
+           long nslab, const Bank<typename C::T>* __restrict__ bk)
+{
+    using K = Pencil<C>;
+    __shared__ cx<typename C::T> sm[C::LDS];
+    const int tid = threadIdx.x;
+    const long bid = blockIdx.x;
+    K::stage(tid, bid, nslab, in, sm);
+    __syncthreads();
+    K::zpass(tid, bid, nslab, sm, bk);
+    __syncthreads();
+    K::ypass(tid, bid, nslab, sm, out, bk);
+}
+
+template <class C>
+__global__ void __launch_bounds__(C::NTB)
+    k_xline(cx<typename C::T>* __restrict__ buf, long nlines,
+            const Bank<typename C::T>* __restrict__ bk, typename C::T scl)
+{
+    Pencil<C>::xline((long)blockIdx.x * C::NTB + threadIdx.x, nlines, buf, bk, scl);
+}
+
+}  // namespace fx
+
+#include <fx_launch.hpp>

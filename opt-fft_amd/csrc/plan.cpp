@@ -1,0 +1,393 @@
+// Plan step and extended C ABI (include/fft3d_mi355x.h).
+//
+// create():
+//   1. filters the kernel registry by geometry, precision and direction;
+//   2. computes the constant bank in long double -- twiddle tables W_N^m for every line
+//      length, the Rader-17 spectra Bhat = DFT16(b')/16 with b'[q] = W17^(3^-q)
+//      (bench/geom/strategies/L17_rader.md:21-55; Bhat[0] = -1/16 and Bhat[8] = sqrt(17)/16
+//      written in closed form) and the folded-17 cos/sin rows -- rounds it to the plan's
+//      precision and uploads it;
+//   3. when several kernels qualify, races them on the device and keeps the fastest: the
+//      race-at-create of bench/geom/best/fft3d_best.c:131-193 (trial: warm executes, then
+//      best of several timed groups), with the verdict cached per process and optionally
+//      in a wisdom file (the per-host cache of bench/gen/impl_14/gen_race.c:45-49).
+// execute() is a single asynchronous launch sequence on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cx.hpp"
+#include "fft3d_mi355x.h"
+#include "plan_internal.hpp"
+#include "registry.hpp"
+
+namespace fx {
+
+static thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+const char* last_error() { return g_err.c_str(); }
+void clear_error() { g_err.clear(); }
+
+// ------------------------------------------------------------------------ race + wisdom
+static std::mutex g_wisdom_mu;
+static std::map<std::string, std::string> g_wisdom;
+static bool g_wisdom_loaded = false;
+
+static void wisdom_load_locked()
+{
+    if (g_wisdom_loaded) return;
+    g_wisdom_loaded = true;
+    const char* path = getenv("FFT3D_MX_WISDOM");
+    if (!path || !*path) return;
+    FILE* f = fopen(path, "r");
+    if (!f) return;
+    char key[256], name[128];
+    while (fscanf(f, "%255s %127s", key, name) == 2) g_wisdom[key] = name;
+    fclose(f);
+}
+
+static void wisdom_store_locked(const std::string& key, const std::string& name)
+{
+    g_wisdom[key] = name;
+    const char* path = getenv("FFT3D_MX_WISDOM");
+    if (!path || !*path) return;
+    FILE* f = fopen(path, "a");
+    if (!f) return;
+    fprintf(f, "%s %s\n", key.c_str(), name.c_str());
+    fclose(f);
+}
+
+static std::string wisdom_key(int n0, int n1, int n2, int prec, int dir, long batch)
+{
+    int dev = 0;
+    hipDeviceProp_t prop;
+    std::string arch = "unknown";
+    int cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+        arch = prop.gcnArchName;
+        cus = prop.multiProcessorCount;
+        for (auto& ch : arch)
+            if (ch == ':' || ch == ' ') ch = '_';
+    }
+    int bclass = 0;
+    while ((1L << (bclass + 1)) <= batch) ++bclass;
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s/%dcu/%dx%dx%d/%s/%s/b2^%d", arch.c_str(), cus, n0, n1, n2,
+             prec == kC128 ? "c128" : "c64", dir < 0 ? "fwd" : "bwd", bclass);
+    return buf;
+}
+
+// Time one candidate: 2 warm launches, then the best of 3 groups of 5.  Seconds per
+// launch, or -1 if it failed to launch.
+static double trial(const KernelDesc& k, const void* din, void* dout, long nvol,
+                    const void* bank, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+{
+    for (int i = 0; i < 2; ++i)
+        if (k.launch(din, dout, nvol, bank, 1.0, s) != hipSuccess) return -1.0;
+    double best = 1e30;
+    for (int g = 0; g < 3; ++g) {
+        if (hipEventRecord(e0, s) != hipSuccess) return -1.0;
+        for (int i = 0; i < 5; ++i)
+            if (k.launch(din, dout, nvol, bank, 1.0, s) != hipSuccess) return -1.0;
+        if (hipEventRecord(e1, s) != hipSuccess) return -1.0;
+        if (hipEventSynchronize(e1) != hipSuccess) return -1.0;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
+        const double per = (double)ms * 1e-3 / 5.0;
+        if (per < best) best = per;
+    }
+    return best;
+}
+
+static const KernelDesc* race(const std::vector<const KernelDesc*>& cands, int n0, int n1,
+                              int n2, int prec, long batch, const void* bank, double* best_s)
+{
+    const size_t vol_bytes = (size_t)n0 * n1 * n2 * elem_bytes(prec);
+    // Cap the race buffers at ~256 MiB each so planning stays bounded; at least one
+    // volume, at most the planned batch.
+    long rb = (long)((256ull << 20) / vol_bytes);
+    if (rb < 1) rb = 1;
+    if (rb > batch) rb = batch;
+    void *din = nullptr, *dout = nullptr;
+    if (hipMalloc(&din, rb * vol_bytes) != hipSuccess) return cands[0];
+    if (hipMalloc(&dout, rb * vol_bytes) != hipSuccess) {
+        (void)hipFree(din);
+        return cands[0];
+    }
+    // 0x3F80_0000 per 32-bit word: 1.0f for complex64, ~0.0078 per double for complex128.
+    (void)hipMemsetD32((hipDeviceptr_t)din, 0x3F800000u, rb * vol_bytes / 4);
+    hipStream_t s;
+    hipEvent_t e0, e1;
+    const KernelDesc* win = cands[0];
+    double wt = 1e30;
+    if (hipStreamCreate(&s) == hipSuccess) {
+        if (hipEventCreate(&e0) == hipSuccess) {
+            if (hipEventCreate(&e1) == hipSuccess) {
+                for (const KernelDesc* k : cands) {
+                    const double t = trial(*k, din, dout, rb, bank, s, e0, e1);
+                    if (t > 0 && t < wt) {
+                        wt = t;
+                        win = k;
+                    }
+                }
+                (void)hipEventDestroy(e1);
+            }
+            (void)hipEventDestroy(e0);
+        }
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+    (void)hipFree(din);
+    (void)hipFree(dout);
+    (void)hipGetLastError();
+    *best_s = wt < 1e29 ? wt * (double)batch / (double)rb : 0.0;
+    return win;
+}
+
+}  // namespace fx
+
+using namespace fx;
+
+struct fft3d_mx_plan {
+    int n0, n1, n2, batch, dir, prec;
+    unsigned flags;
+    const KernelDesc* k;
+    void* bank;
+    double scale;
+    double tune_us;
+    int device;
+};
+
+extern "C" const char* fft3d_mx_last_error(void) { return last_error(); }
+extern "C" void fft3d_mx_clear_error(void) { clear_error(); }
+
+static void candidates(int n0, int n1, int n2, int prec, int dir,
+                       std::vector<const KernelDesc*>& out)
+{
+    for (const KernelDesc& k : registry())
+        if (k.n0 == n0 && k.n1 == n1 && k.n2 == n2 && k.prec == prec && k.dir == dir)
+            out.push_back(&k);
+}
+
+extern "C" int fft3d_mx_supports(int n0, int n1, int n2, int precision)
+{
+    std::vector<const KernelDesc*> c;
+    candidates(n0, n1, n2, precision, -1, c);
+    return !c.empty();
+}
+
+extern "C" size_t fft3d_mx_volume_bytes(int n0, int n1, int n2, int precision)
+{
+    if (n0 <= 0 || n1 <= 0 || n2 <= 0 || (precision != kC128 && precision != kC64)) return 0;
+    return (size_t)n0 * n1 * n2 * elem_bytes(precision);
+}
+
+static fft3d_mx_plan* create_impl(int n0, int n1, int n2, int batch, int direction,
+                                  int precision, unsigned flags, const char* forced)
+{
+    if (n0 <= 0 || n1 <= 0 || n2 <= 0 || batch < 0) {
+        fail(FFT3D_MX_EINVAL, "fft3d_mx_create: bad extents %dx%dx%d or batch %d", n0, n1, n2,
+             batch);
+        return nullptr;
+    }
+    if (direction != -1 && direction != 1) {
+        fail(FFT3D_MX_EINVAL, "fft3d_mx_create: direction must be -1 or +1, got %d", direction);
+        return nullptr;
+    }
+    if (precision != kC128 && precision != kC64) {
+        fail(FFT3D_MX_EINVAL, "fft3d_mx_create: unknown precision %d", precision);
+        return nullptr;
+    }
+    std::vector<const KernelDesc*> cands;
+    candidates(n0, n1, n2, precision, direction, cands);
+    if (cands.empty()) {
+        fail(FFT3D_MX_EUNSUPPORTED, "fft3d_mx_create: no kernel for %dx%dx%d %s", n0, n1, n2,
+             precision == kC128 ? "complex128" : "complex64");
+        return nullptr;
+    }
+    if (!forced || !*forced) forced = getenv("FFT3D_MX_KERNEL");
+    const KernelDesc* pick = nullptr;
+    if (forced && *forced) {
+        for (const KernelDesc* k : cands)
+            if (!strcmp(k->name, forced)) pick = k;
+        if (!pick && getenv("FFT3D_MX_KERNEL") != forced) {
+            fail(FFT3D_MX_EUNSUPPORTED, "fft3d_mx_create: kernel '%s' does not serve %dx%dx%d",
+                 forced, n0, n1, n2);
+            return nullptr;
+        }
+    }
+
+    fft3d_mx_plan* p = new (std::nothrow) fft3d_mx_plan();
+    if (!p) {
+        fail(FFT3D_MX_ENOMEM, "fft3d_mx_create: out of host memory");
+        return nullptr;
+    }
+    p->n0 = n0;
+    p->n1 = n1;
+    p->n2 = n2;
+    p->batch = batch;
+    p->dir = direction;
+    p->prec = precision;
+    p->flags = flags;
+    p->scale = (flags & FFT3D_MX_NORMALIZE) ? 1.0 / ((double)n0 * n1 * n2) : 1.0;
+    p->tune_us = 0.0;
+    if (hipGetDevice(&p->device) != hipSuccess) p->device = 0;
+
+    std::vector<unsigned char> img;
+    bank_host_image(precision, img);
+    hipError_t e = hipMalloc(&p->bank, img.size());
+    if (e != hipSuccess) {
+        fail(FFT3D_MX_ENOMEM, "fft3d_mx_create: hipMalloc(bank): %s", hipGetErrorString(e));
+        delete p;
+        return nullptr;
+    }
+    e = hipMemcpy(p->bank, img.data(), img.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        fail(FFT3D_MX_EHIP, "fft3d_mx_create: upload bank: %s", hipGetErrorString(e));
+        (void)hipFree(p->bank);
+        delete p;
+        return nullptr;
+    }
+
+    if (!pick) {
+        if (cands.size() == 1 || (flags & FFT3D_MX_NO_TUNE) || batch == 0) {
+            pick = cands[0];
+        } else {
+            const std::string key = wisdom_key(n0, n1, n2, precision, direction, batch);
+            std::lock_guard<std::mutex> g(g_wisdom_mu);
+            wisdom_load_locked();
+            auto it = g_wisdom.find(key);
+            if (it != g_wisdom.end())
+                for (const KernelDesc* k : cands)
+                    if (it->second == k->name) pick = k;
+            if (!pick) {
+                double best_s = 0.0;
+                pick = race(cands, n0, n1, n2, precision, batch, p->bank, &best_s);
+                p->tune_us = best_s * 1e6;
+                wisdom_store_locked(key, pick->name);
+            }
+        }
+    }
+    p->k = pick;
+    return p;
+}
+
+extern "C" fft3d_mx_plan* fft3d_mx_create(int n0, int n1, int n2, int batch, int direction,
+                                          int precision, unsigned flags)
+{
+    return create_impl(n0, n1, n2, batch, direction, precision, flags, nullptr);
+}
+
+extern "C" fft3d_mx_plan* fft3d_mx_create_named(int n0, int n1, int n2, int batch,
+                                                int direction, int precision, unsigned flags,
+                                                const char* kernel)
+{
+    if (!kernel || !*kernel) {
+        fail(FFT3D_MX_EINVAL, "fft3d_mx_create_named: empty kernel name");
+        return nullptr;
+    }
+    return create_impl(n0, n1, n2, batch, direction, precision, flags, kernel);
+}
+
+extern "C" int fft3d_mx_execute_n(fft3d_mx_plan* p, const void* in, void* out, long batch,
+                                  void* stream)
+{
+    if (!p) return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute: NULL plan");
+    if (batch < 0 || batch > p->batch)
+        return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute: batch %ld outside [0, %d]", batch,
+                    p->batch);
+    if (batch == 0) return FFT3D_MX_OK;
+    if (!in || !out) return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute: NULL buffer");
+    if (in == (const void*)out)
+        return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute: in and out must be distinct");
+    int cur = p->device;
+    (void)hipGetDevice(&cur);
+    if (cur != p->device) (void)hipSetDevice(p->device);
+    const hipError_t e = p->k->launch(in, out, batch, p->bank, p->scale, (hipStream_t)stream);
+    if (cur != p->device) (void)hipSetDevice(cur);
+    if (e != hipSuccess)
+        return fail(FFT3D_MX_EHIP, "fft3d_mx_execute: %s: %s", p->k->name, hipGetErrorString(e));
+    return FFT3D_MX_OK;
+}
+
+extern "C" int fft3d_mx_execute(fft3d_mx_plan* p, const void* in, void* out, void* stream)
+{
+    if (!p) return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute: NULL plan");
+    return fft3d_mx_execute_n(p, in, out, p->batch, stream);
+}
+
+extern "C" void fft3d_mx_destroy(fft3d_mx_plan* p)
+{
+    if (!p) return;
+    (void)hipFree(p->bank);
+    delete p;
+}
+
+extern "C" int fft3d_mx_plan_get_info(const fft3d_mx_plan* p, fft3d_mx_plan_info* info)
+{
+    if (!p || !info) return fail(FFT3D_MX_EINVAL, "fft3d_mx_plan_get_info: NULL argument");
+    std::memset(info, 0, sizeof *info);
+    info->n0 = p->n0;
+    info->n1 = p->n1;
+    info->n2 = p->n2;
+    info->batch = p->batch;
+    info->direction = p->dir;
+    info->precision = p->prec;
+    info->kind = p->k->kind;
+    info->volumes_per_wg = p->k->vpw;
+    info->threads = p->k->nthreads;
+    info->variant = p->k->variant;
+    info->lds_bytes = p->k->lds_bytes;
+    info->tune_us = p->tune_us;
+    snprintf(info->kernel, sizeof info->kernel, "%s", p->k->name);
+    return FFT3D_MX_OK;
+}
+
+extern "C" int fft3d_mx_kernel_count(void) { return (int)registry().size(); }
+
+extern "C" const char* fft3d_mx_kernel_name(int i, int* n0, int* n1, int* n2, int* precision,
+                                            int* direction)
+{
+    if (i < 0 || i >= (int)registry().size()) return nullptr;
+    const KernelDesc& k = registry()[i];
+    if (n0) *n0 = k.n0;
+    if (n1) *n1 = k.n1;
+    if (n2) *n2 = k.n2;
+    if (precision) *precision = k.prec;
+    if (direction) *direction = k.dir;
+    return k.name;
+}
+
+// Bank image for tests: lets the CPU suite check the plan-time tables without a GPU.
+extern "C" int fft3d_mx_bank_image(int precision, void* dst, size_t cap, size_t* size)
+{
+    if (precision != kC128 && precision != kC64)
+        return fail(FFT3D_MX_EINVAL, "fft3d_mx_bank_image: unknown precision %d", precision);
+    std::vector<unsigned char> img;
+    bank_host_image(precision, img);
+    if (size) *size = img.size();
+    if (dst) {
+        if (cap < img.size()) return fail(FFT3D_MX_EINVAL, "fft3d_mx_bank_image: buffer too small");
+        std::memcpy(dst, img.data(), img.size());
+    }
+    return FFT3D_MX_OK;
+}

@@ -1,0 +1,257 @@
+"""Python host side of the MI355X batched 3D FFT: ctypes bindings of
+``lib/libfft3d_mi355x.so`` and a numpy-facing mirror of the reference's transform interface.
+
+Mirrored interface (python/fft3d.py:360-423): ``fftn`` / ``ifftn`` over the last three
+axes, ``fft3d`` / ``ifft3d`` for (B, n0, n1, n2) batches, the same sign and normalisation
+conventions (forward unnormalised, ``ifftn`` carries 1/V, python/slow_dft.py:11-18) and the
+same guard rails for bad input (ValueError).  Two deliberate differences:
+
+* only the geometries the GPU kernels serve are accepted (``supports``); anything else
+  raises ValueError instead of falling back to a slower route -- there is no CPU path;
+* complex64 input is transformed in single precision on the GPU (the reference's Python
+  module upcasts to double internally); complex128 stays complex128.
+
+Every call goes through the C ABI of the shared library; nothing here computes a transform.
+The library must be built (``make -C opt-fft_amd``); a missing library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import hip
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfft3d_mi355x.so")
+
+FORWARD, BACKWARD = -1, +1
+C128, C64 = 0, 1
+NORMALIZE, NO_TUNE = 1, 2
+
+# Every symbol the three public headers declare (include/*.h).
+EXPORTED = [
+    "fft3d_name", "fft3d_description", "fft3d_supports", "fft3d_create", "fft3d_execute",
+    "fft3d_destroy",
+    "fft3d_gpu_name", "fft3d_gpu_description", "fft3d_gpu_supports", "fft3d_gpu_create",
+    "fft3d_gpu_execute", "fft3d_gpu_destroy",
+    "fft3d_mx_last_error", "fft3d_mx_clear_error", "fft3d_mx_supports", "fft3d_mx_create",
+    "fft3d_mx_create_named", "fft3d_mx_execute", "fft3d_mx_execute_n", "fft3d_mx_destroy",
+    "fft3d_mx_plan_get_info", "fft3d_mx_kernel_count", "fft3d_mx_kernel_name",
+    "fft3d_mx_bank_image", "fft3d_mx_volume_bytes",
+]
+
+
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("n0", ctypes.c_int), ("n1", ctypes.c_int), ("n2", ctypes.c_int),
+                ("batch", ctypes.c_int), ("direction", ctypes.c_int),
+                ("precision", ctypes.c_int), ("kind", ctypes.c_int),
+                ("volumes_per_wg", ctypes.c_int), ("threads", ctypes.c_int),
+                ("variant", ctypes.c_int), ("lds_bytes", ctypes.c_size_t),
+                ("tune_us", ctypes.c_double), ("kernel", ctypes.c_char * 96)]
+
+
+_lib = None
+
+
+def load():
+    """Load the shared library (raises OSError if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with `make -C opt-fft_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, ci, cu, cl, cs = (ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_long,
+                              ctypes.c_size_t)
+        ip = ctypes.POINTER(ci)
+        L.fft3d_name.restype = ctypes.c_char_p
+        L.fft3d_description.restype = ctypes.c_char_p
+        L.fft3d_supports.argtypes = [ci]
+        L.fft3d_create.argtypes = [ci, ci]
+        L.fft3d_create.restype = vp
+        L.fft3d_execute.argtypes = [vp, vp, vp]
+        L.fft3d_destroy.argtypes = [vp]
+        L.fft3d_gpu_name.restype = ctypes.c_char_p
+        L.fft3d_gpu_description.restype = ctypes.c_char_p
+        L.fft3d_gpu_supports.argtypes = [ci]
+        L.fft3d_gpu_create.argtypes = [ci, ci]
+        L.fft3d_gpu_create.restype = vp
+        L.fft3d_gpu_execute.argtypes = [vp, vp, vp]
+        L.fft3d_gpu_destroy.argtypes = [vp]
+        L.fft3d_mx_last_error.restype = ctypes.c_char_p
+        L.fft3d_mx_supports.argtypes = [ci, ci, ci, ci]
+        L.fft3d_mx_create.argtypes = [ci, ci, ci, ci, ci, ci, cu]
+        L.fft3d_mx_create.restype = vp
+        L.fft3d_mx_create_named.argtypes = [ci, ci, ci, ci, ci, ci, cu, ctypes.c_char_p]
+        L.fft3d_mx_create_named.restype = vp
+        L.fft3d_mx_execute.argtypes = [vp, vp, vp, vp]
+        L.fft3d_mx_execute_n.argtypes = [vp, vp, vp, cl, vp]
+        L.fft3d_mx_destroy.argtypes = [vp]
+        L.fft3d_mx_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
+        L.fft3d_mx_kernel_name.argtypes = [ci, ip, ip, ip, ip, ip]
+        L.fft3d_mx_kernel_name.restype = ctypes.c_char_p
+        L.fft3d_mx_bank_image.argtypes = [ci, vp, cs, ctypes.POINTER(cs)]
+        L.fft3d_mx_volume_bytes.argtypes = [ci, ci, ci, ci]
+        L.fft3d_mx_volume_bytes.restype = cs
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return load().fft3d_mx_last_error().decode()
+
+
+def kernels():
+    """[(name, (n0,n1,n2), precision, direction)] for every compiled kernel variant."""
+    L = load()
+    out = []
+    a = [ctypes.c_int() for _ in range(5)]
+    for i in range(L.fft3d_mx_kernel_count()):
+        name = L.fft3d_mx_kernel_name(i, *[ctypes.byref(x) for x in a])
+        out.append((name.decode(), (a[0].value, a[1].value, a[2].value), a[3].value,
+                    a[4].value))
+    return out
+
+
+def supports(shape, precision=C128):
+    n0, n1, n2 = (int(n) for n in shape)
+    return bool(load().fft3d_mx_supports(n0, n1, n2, precision))
+
+
+def bank_image(precision=C128):
+    """The plan-time constant bank as a flat float array (see fft3d_mx_bank_image)."""
+    L = load()
+    size = ctypes.c_size_t(0)
+    if L.fft3d_mx_bank_image(precision, None, 0, ctypes.byref(size)) != 0:
+        raise ValueError(last_error())
+    buf = np.empty(size.value, dtype=np.uint8)
+    if L.fft3d_mx_bank_image(precision, buf.ctypes.data, buf.nbytes, ctypes.byref(size)) != 0:
+        raise ValueError(last_error())
+    return buf.view(np.float64 if precision == C128 else np.float32)
+
+
+def _prec_of(dtype):
+    dtype = np.dtype(dtype)
+    if dtype == np.complex128:
+        return C128
+    if dtype == np.complex64:
+        return C64
+    raise ValueError(f"dtype {dtype} is not complex64/complex128")
+
+
+class Plan:
+    """One fft3d_mx plan: (n0,n1,n2), batch, direction, precision, flags."""
+
+    def __init__(self, shape, batch, direction=FORWARD, precision=C128, flags=0, kernel=None):
+        L = load()
+        self.shape = tuple(int(n) for n in shape)
+        self.batch, self.direction, self.precision = int(batch), int(direction), int(precision)
+        self.flags = int(flags)
+        if kernel is None:
+            h = L.fft3d_mx_create(*self.shape, self.batch, self.direction, self.precision,
+                                  self.flags)
+        else:
+            h = L.fft3d_mx_create_named(*self.shape, self.batch, self.direction,
+                                        self.precision, self.flags, kernel.encode())
+        if not h:
+            raise ValueError(last_error())
+        self.handle = ctypes.c_void_p(h)
+        self.elem = 16 if self.precision == C128 else 8
+        self.dtype = np.complex128 if self.precision == C128 else np.complex64
+        self.volume = self.shape[0] * self.shape[1] * self.shape[2]
+
+    def info(self):
+        inf = PlanInfo()
+        if load().fft3d_mx_plan_get_info(self.handle, ctypes.byref(inf)) != 0:
+            raise ValueError(last_error())
+        return inf
+
+    @property
+    def kernel(self):
+        return self.info().kernel.decode()
+
+    def execute(self, din, dout, batch=None, stream=None):
+        """Enqueue on `stream` (hip.Stream, raw handle or None) with device addresses."""
+        s = stream.handle if isinstance(stream, hip.Stream) else stream
+        n = self.batch if batch is None else int(batch)
+        rc = load().fft3d_mx_execute_n(self.handle, din, dout, n, s)
+        if rc != 0:
+            raise RuntimeError(last_error())
+
+    def run(self, x):
+        """Host convenience: upload x (B, n0, n1, n2), transform, synchronize, download."""
+        x = np.ascontiguousarray(x, dtype=self.dtype)
+        if x.shape[1:] != self.shape or x.shape[0] > self.batch:
+            raise ValueError(f"input shape {x.shape} does not fit plan {self.shape} x {self.batch}")
+        nbytes = x.nbytes
+        din, dout = hip.DeviceBuffer(nbytes), hip.DeviceBuffer(nbytes)
+        try:
+            din.upload(x)
+            self.execute(din.addr, dout.addr, batch=x.shape[0])
+            hip.synchronize()
+            return dout.download(self.dtype, x.shape)
+        finally:
+            din.close()
+            dout.close()
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            load().fft3d_mx_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _transform(x, direction, normalize):
+    x = np.asarray(x)
+    if x.ndim < 3:
+        raise ValueError(f"need at least 3 dimensions, got shape {x.shape}")
+    if np.iscomplexobj(x) and x.dtype == np.complex64:
+        dtype = np.complex64
+    else:
+        dtype = np.complex128
+    shape = x.shape[-3:]
+    if min(shape) == 0:
+        raise ValueError(f"cannot transform a length-0 axis: shape {x.shape}")
+    lead = x.shape[:-3]
+    batch = int(np.prod(lead)) if lead else 1
+    if batch == 0:
+        return np.zeros(x.shape, dtype=dtype)
+    prec = _prec_of(dtype)
+    if not supports(shape, prec):
+        raise ValueError(f"no MI355X kernel for {shape} at {np.dtype(dtype).name}")
+    flat = np.ascontiguousarray(x.reshape((batch,) + shape), dtype=dtype)
+    plan = Plan(shape, batch, direction, prec, NORMALIZE if normalize else 0)
+    try:
+        y = plan.run(flat)
+    finally:
+        plan.close()
+    return y.reshape(x.shape)
+
+
+def fftn(x):
+    """Forward, unnormalised 3D DFT over the last three axes (python/fft3d.py:360)."""
+    return _transform(x, FORWARD, False)
+
+
+def ifftn(x):
+    """Backward 3D DFT over the last three axes with 1/V (python/fft3d.py:402)."""
+    return _transform(x, BACKWARD, True)
+
+
+def fft3d(x):
+    """(B, n0, n1, n2) or (n0, n1, n2) forward transform."""
+    x = np.asarray(x)
+    if x.ndim not in (3, 4):
+        raise ValueError(f"fft3d expects 3 or 4 dimensions, got {x.shape}")
+    return fftn(x)
+
+
+def ifft3d(x):
+    x = np.asarray(x)
+    if x.ndim not in (3, 4):
+        raise ValueError(f"ifft3d expects 3 or 4 dimensions, got {x.shape}")
+    return ifftn(x)

@@ -1,0 +1,125 @@
+// TEST INFRASTRUCTURE: host emulation of the kernel launchers.
+//
+// Found on the include path before opt-fft_amd/csrc when the emulator library is built, so
+// the kernel translation units register these host loops instead of device launches.  Each
+// block runs its phases in barrier order; within a phase the threads run one after another
+// in ascending order, or descending when fx::g_emu_reverse is set -- a phase whose result
+// depends on that order has an intra-phase race.  LDS starts out NaN-filled (any read of a
+// slot no thread wrote poisons the output) and is flanked by guard elements that must
+// survive every block (an out-of-range LDS write is reported as -1000 - block).
+#pragma once
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace fx {
+
+extern int g_emu_reverse;
+extern long g_emu_bad_block;
+
+template <typename T>
+struct EmuLds {
+    static constexpr int kGuard = 64;
+    std::vector<cx<T>> buf;
+    explicit EmuLds(int n) : buf(n + 2 * kGuard) {}
+    cx<T>* data() { return buf.data() + kGuard; }
+    void poison()
+    {
+        for (auto& e : buf) e = {T(NAN), T(NAN)};
+        for (int g = 0; g < kGuard; ++g) {
+            buf[g] = {T(-7777), T(7777)};
+            buf[buf.size() - 1 - g] = {T(-7777), T(7777)};
+        }
+    }
+    bool guards_ok() const
+    {
+        for (int g = 0; g < kGuard; ++g) {
+            const cx<T>& a = buf[g];
+            const cx<T>& b = buf[buf.size() - 1 - g];
+            if (a.r != T(-7777) || a.i != T(7777) || b.r != T(-7777) || b.i != T(7777))
+                return false;
+        }
+        return true;
+    }
+};
+
+template <typename F>
+inline void emu_threads(int nt, F f)
+{
+    if (g_emu_reverse)
+        for (int t = nt - 1; t >= 0; --t) f(t);
+    else
+        for (int t = 0; t < nt; ++t) f(t);
+}
+
+template <class C>
+hipError_t launch_fused(const void* in, void* out, long nvol, const void* bank, double scl,
+                        hipStream_t)
+{
+    using T = typename C::T;
+    using K = Fused<C>;
+    if (nvol <= 0) return hipSuccess;
+    const long grid = (nvol + C::VPW - 1) / C::VPW;
+    const Bank<T>* bk = (const Bank<T>*)bank;
+    EmuLds<T> lds(C::LDS);
+    for (long b = 0; b < grid; ++b) {
+        lds.poison();
+        cx<T>* sm = lds.data();
+        emu_threads(C::NT, [&](int t) { K::stage(t, b, nvol, (const cx<T>*)in, sm); });
+        emu_threads(C::NT, [&](int t) { K::zpass(t, b, nvol, sm, bk); });
+        emu_threads(C::NT, [&](int t) { K::ypass(t, b, nvol, sm, bk); });
+        emu_threads(C::NT, [&](int t) { K::xpass(t, b, nvol, sm, (cx<T>*)out, bk, (T)scl); });
+        if (!lds.guards_ok() && g_emu_bad_block < 0) g_emu_bad_block = b;
+    }
+    return hipSuccess;
+}
+
+template <class C>
+hipError_t launch_pencil(const void* in, void* out, long nvol, const void* bank, double scl,
+                         hipStream_t)
+{
+    using T = typename C::T;
+    using K = Pencil<C>;
+    if (nvol <= 0) return hipSuccess;
+    const Bank<T>* bk = (const Bank<T>*)bank;
+    EmuLds<T> lds(C::LDS);
+    const long chunk = C::CHUNK > 0 ? C::CHUNK : nvol;
+    for (long v0 = 0; v0 < nvol; v0 += chunk) {
+        const long nv = (nvol - v0) < chunk ? (nvol - v0) : chunk;
+        const cx<T>* src = (const cx<T>*)in + v0 * C::V;
+        cx<T>* dst = (cx<T>*)out + v0 * C::V;
+        const long nslab = nv * C::N0;
+        const long ga = (nslab + C::SPW - 1) / C::SPW;
+        for (long b = 0; b < ga; ++b) {
+            lds.poison();
+            cx<T>* sm = lds.data();
+            emu_threads(C::NT, [&](int t) { K::stage(t, b, nslab, src, sm); });
+            emu_threads(C::NT, [&](int t) { K::zpass(t, b, nslab, sm, bk); });
+            emu_threads(C::NT, [&](int t) { K::ypass(t, b, nslab, sm, dst, bk); });
+            if (!lds.guards_ok() && g_emu_bad_block < 0) g_emu_bad_block = b;
+        }
+        const long nlines = nv * C::PLANE;
+        const long gb = (nlines + C::NTB - 1) / C::NTB;
+        for (long b = 0; b < gb; ++b)
+            emu_threads(C::NTB, [&](int t) { K::xline(b * C::NTB + t, nlines, dst, bk, (T)scl); });
+    }
+    return hipSuccess;
+}
+
+template <class C>
+KernelDesc fused_desc(const char* name)
+{
+    return KernelDesc{C::N0, C::N1, C::N2, prec_of<typename C::T>(), C::S, kFused, C::VPW,
+                      C::NT, C::V17, sizeof(cx<typename C::T>) * (size_t)C::LDS, name,
+                      &launch_fused<C>};
+}
+
+template <class C>
+KernelDesc pencil_desc(const char* name)
+{
+    return KernelDesc{C::N0, C::N1, C::N2, prec_of<typename C::T>(), C::S, kPencil, C::SPW,
+                      C::NT, 0, sizeof(cx<typename C::T>) * (size_t)C::LDS, name,
+                      &launch_pencil<C>};
+}
+
+}  // namespace fx

@@ -1,0 +1,177 @@
+"""GPU parity: every compiled kernel variant against the oracle, through the C ABI.
+
+Tolerances (relative L2 over the whole batch, bench/geom/check.py:29-30):
+  complex128  1e-13   (observed ~5e-16)
+  complex64   2e-6    (single precision: ~1e-7 observed; oracle is exact long double)
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = {0: 1e-13, 1: 2e-6}
+
+
+def _kernels(pkg):
+    return pkg.fft3d.kernels()
+
+
+def test_every_kernel_variant_matches_oracle(gpu):
+    f = gpu.fft3d
+    ks = _kernels(gpu)
+    assert len(ks) >= 60
+    worst = {}
+    for name, shape, prec, direction in ks:
+        dt = np.complex128 if prec == f.C128 else np.complex64
+        # batch sizes: one volume, and a batch that leaves the last workgroup partial
+        for batch in (1, 5):
+            x = oracle.make_input(shape, batch, seed=101 + batch, dist="uniform", dtype=dt)
+            plan = f.Plan(shape, batch, direction, prec, kernel=name)
+            assert plan.kernel == name
+            y = plan.run(x)
+            plan.close()
+            ref = oracle.fftn_numpy(x, sign=direction)
+            err = oracle.rel_l2(y, ref)
+            print(f"{name} B={batch} rel_l2={err:.2e}", flush=True)
+            assert np.all(np.isfinite(y)), name
+            assert err <= TOL[prec], f"{name} B={batch}: rel_l2 {err:.3e}"
+            worst[name] = max(worst.get(name, 0.0), err)
+
+
+def test_long_double_oracle_agrees_at_small_sizes(gpu):
+    """Same comparison against the long-double dense DFT (independent of numpy)."""
+    f = gpu.fft3d
+    for shape in [(6, 6, 6), (8, 8, 8), (12, 12, 12), (17, 17, 17), (8, 8, 16)]:
+        for direction in (f.FORWARD, f.BACKWARD):
+            x = oracle.make_input(shape, 2, seed=5, dist="normal_half")
+            plan = f.Plan(shape, 2, direction, f.C128)
+            y = plan.run(x)
+            plan.close()
+            err = oracle.rel_l2(y, oracle.dft3d(x, sign=direction))
+            assert err <= 1e-13, f"{shape} dir={direction}: {err:.3e}"
+
+
+@pytest.mark.parametrize("shape,batch,dist", [
+    ((8, 8, 8), 4096, "uniform"),
+    ((17, 17, 17), 1024, "uniform"),
+    ((36, 36, 36), 128, "normal_half"),
+    ((24, 24, 48), 64, "normal_half"),
+])
+def test_workload_sizes_forward_and_inverse(gpu, shape, batch, dist):
+    """Full workload sizes: forward vs numpy.fft, and backward(normalised) round trip."""
+    f = gpu.fft3d
+    x = oracle.make_input(shape, batch, seed=1000 + batch, dist=dist)
+    fwd = f.Plan(shape, batch, f.FORWARD, f.C128)
+    y = fwd.run(x)
+    err = oracle.rel_l2(y, oracle.fftn_numpy(x))
+    print(f"{shape} B={batch} {fwd.kernel}: forward rel_l2={err:.2e}", flush=True)
+    assert err <= 1e-13
+    bwd = f.Plan(shape, batch, f.BACKWARD, f.C128, flags=f.NORMALIZE)
+    back = bwd.run(y)
+    rt = oracle.rel_l2(back, x)
+    assert rt <= 1e-13, f"round trip {rt:.3e}"
+    # inverse against numpy.ifftn directly
+    inv = bwd.run(x)
+    assert oracle.rel_l2(inv, np.fft.ifftn(x, axes=(-3, -2, -1))) <= 1e-13
+    fwd.close()
+    bwd.close()
+
+
+def test_repeatable_and_input_untouched(gpu):
+    f = gpu.fft3d
+    h = gpu.hip
+    shape, batch = (17, 17, 17), 7
+    x = oracle.make_input(shape, batch, seed=3)
+    plan = f.Plan(shape, batch)
+    din, dout = h.DeviceBuffer(x.nbytes), h.DeviceBuffer(x.nbytes)
+    din.upload(x)
+    plan.execute(din.addr, dout.addr)
+    h.synchronize()
+    a = dout.download(np.complex128, x.shape)
+    dout.fill_zero()
+    plan.execute(din.addr, dout.addr)
+    h.synchronize()
+    b = dout.download(np.complex128, x.shape)
+    assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), "not repeatable"
+    assert np.array_equal(din.download(np.complex128, x.shape), x), "input modified"
+    # anti-memoisation: perturb one element, the output must change (bench/geom/driver.c:225-239)
+    x2 = x.copy()
+    x2.reshape(-1)[0] += 1.0 - 2.0j
+    din.upload(x2)
+    plan.execute(din.addr, dout.addr)
+    h.synchronize()
+    c = dout.download(np.complex128, x.shape)
+    assert not np.array_equal(a, c)
+    assert oracle.rel_l2(c, oracle.fftn_numpy(x2)) <= 1e-13
+    din.close()
+    dout.close()
+    plan.close()
+
+
+def test_host_pointer_abi(gpu):
+    """The six fft3d_* symbols of bench/geom/fft3d_api.h on host buffers."""
+    import ctypes
+    L = gpu.fft3d.load()
+    for n in (6, 8, 12, 17, 24, 36):
+        assert L.fft3d_supports(n)
+        x = oracle.make_input((n, n, n), 2, seed=n)
+        out = np.zeros_like(x)
+        p = L.fft3d_create(n, 2)
+        assert p
+        L.fft3d_execute(p, x.ctypes.data, out.ctypes.data)
+        assert gpu.fft3d.last_error() == ""
+        L.fft3d_destroy(p)
+        assert oracle.rel_l2(out, oracle.fftn_numpy(x)) <= 1e-13, n
+    assert not L.fft3d_supports(7)
+    assert not L.fft3d_create(7, 1)
+    assert not L.fft3d_create(8, 0)
+    assert ctypes.c_char_p(L.fft3d_name()).value
+
+
+def test_device_pointer_abi(gpu):
+    """The six fft3d_gpu_* symbols of bench/gpu/fft3d_gpu_api.h on device buffers."""
+    L = gpu.fft3d.load()
+    h = gpu.hip
+    for n in (8, 17, 36):
+        x = oracle.make_input((n, n, n), 3, seed=n + 1)
+        din, dout = h.DeviceBuffer(x.nbytes), h.DeviceBuffer(x.nbytes)
+        din.upload(x)
+        p = L.fft3d_gpu_create(n, 3)
+        assert p
+        L.fft3d_gpu_execute(p, din.addr, dout.addr)
+        h.synchronize()
+        assert h.lib().hipGetLastError() == 0
+        y = dout.download(np.complex128, x.shape)
+        L.fft3d_gpu_destroy(p)
+        din.close()
+        dout.close()
+        assert oracle.rel_l2(y, oracle.fftn_numpy(x)) <= 1e-13, n
+
+
+def test_execute_errors_are_reported(gpu):
+    f = gpu.fft3d
+    plan = f.Plan((8, 8, 8), 4)
+    with pytest.raises(RuntimeError):
+        plan.execute(0, 0)
+    with pytest.raises(RuntimeError):
+        plan.execute(1 << 20, 1 << 21, batch=5)
+    plan.close()
+    with pytest.raises(ValueError):
+        f.Plan((7, 7, 7), 1)
+
+
+def test_mirror_fftn_ifftn(gpu):
+    """Host mirror of python/fft3d.py: fftn/ifftn over the last three axes, dtype kept."""
+    f = gpu.fft3d
+    x = oracle.make_input((12, 12, 12), 3, seed=9)
+    assert oracle.rel_l2(f.fftn(x), np.fft.fftn(x, axes=(-3, -2, -1))) <= 1e-13
+    assert oracle.rel_l2(f.ifftn(f.fftn(x)), x) <= 1e-13
+    x32 = x.astype(np.complex64)
+    y32 = f.fftn(x32)
+    assert y32.dtype == np.complex64
+    assert oracle.rel_l2(y32, oracle.dft3d(x32)) <= 2e-6
+    # a leading batch of more than one axis rides along
+    x5 = oracle.make_input((8, 8, 8), 6, seed=2).reshape(2, 3, 8, 8, 8)
+    assert oracle.rel_l2(f.fftn(x5), np.fft.fftn(x5, axes=(-3, -2, -1))) <= 1e-13
