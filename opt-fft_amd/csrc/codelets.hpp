@@ -35,6 +35,22 @@ constexpr int pow_mod(int g, int e, int m)
     return r;
 }
 
+// perm17.p[u] = 3^u mod 17: the generator powers Rader-17 gathers and scatters by.
+struct Perm17 {
+    int p[16];
+};
+constexpr Perm17 make_perm17()
+{
+    Perm17 r{};
+    int v = 1;
+    for (int u = 0; u < 16; ++u) {
+        r.p[u] = v;
+        v = (v * 3) % 17;
+    }
+    return r;
+}
+constexpr Perm17 kPerm17 = make_perm17();
+
 template <int N, int S, typename T, int V17 = 0>
 FX_HD void dft(cx<T>* x, const Bank<T>* __restrict__ bk);
 
@@ -150,7 +166,7 @@ FX_HD void rader17(cx<T>* x, const Bank<T>* __restrict__ bk)
 {
     cx<T> a[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) a[u] = x[pow_mod(3, u, 17)];
+    for (int u = 0; u < 16; ++u) a[u] = x[kPerm17.p[u]];
     const cx<T> x0 = x[0];
     dft<16, -1, T>(a, bk);
     const cx<T> X0 = x0 + a[0];
@@ -164,7 +180,7 @@ FX_HD void rader17(cx<T>* x, const Bank<T>* __restrict__ bk)
     dft<16, -1, T>(a, bk);
     x[0] = X0;
 #pragma unroll
-    for (int w = 0; w < 16; ++w) x[pow_mod(3, w, 17)] = a[w];
+    for (int w = 0; w < 16; ++w) x[kPerm17.p[w]] = a[w];
 }
 
 // Folded dense 17: X_k = A_k + S i B_k, X_{17-k} = A_k - S i B_k with
@@ -198,6 +214,52 @@ FX_HD void fold17(cx<T>* x, const Bank<T>* __restrict__ bk)
         x[17 - k] = A - sb;
     }
     x[0] = X0;
+}
+
+// Folded dense 17 that hands each output to put(k, X_k) as soon as it is final, so a
+// caller can store it immediately and the 17 results never coexist with u and v.
+template <int S, typename T, typename Put>
+FX_HD void fold17_put(const cx<T>* x, const Bank<T>* __restrict__ bk, Put put)
+{
+    cx<T> u[9], v[9];
+    const cx<T> x0 = x[0];
+    cx<T> X0 = x0;
+#pragma unroll
+    for (int j = 1; j <= 8; ++j) {
+        u[j] = x[j] + x[17 - j];
+        v[j] = x[j] - x[17 - j];
+        X0 = X0 + u[j];
+    }
+    put(0, X0);
+#pragma unroll
+    for (int k = 1; k <= 8; ++k) {
+        cx<T> A = x0, B = {T(0), T(0)};
+#pragma unroll
+        for (int j = 1; j <= 8; ++j) {
+            const int m = (j * k) % 17;
+            const T c = bk->cos17[m], s = bk->sin17[m];
+            A.r += c * u[j].r;
+            A.i += c * u[j].i;
+            B.r += s * v[j].r;
+            B.i += s * v[j].i;
+        }
+        const cx<T> sb = mul_si<S>(B);
+        put(k, A + sb);
+        put(17 - k, A - sb);
+    }
+}
+
+// dft followed by put(k, X_k) for every k; the folded 17 interleaves the two.
+template <int N, int S, typename T, int V17, typename Put>
+FX_HD void dft_put(cx<T>* x, const Bank<T>* __restrict__ bk, Put put)
+{
+    if constexpr (N == 17 && V17 == 1) {
+        fold17_put<S>(x, bk, put);
+    } else {
+        dft<N, S, T, V17>(x, bk);
+#pragma unroll
+        for (int k = 0; k < N; ++k) put(k, x[k]);
+    }
 }
 
 template <int N, int S, typename T, int V17>

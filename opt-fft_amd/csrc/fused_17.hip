@@ -1,7 +1,7 @@
 // LDS-resident fused kernels for L = 17: one 78,608 B complex128 volume per workgroup (two
 // workgroups per CU in 160 KB of LDS), or two complex64 volumes.  Odd extents make every
 // strided line read conflict-free without padding.  Variants: Rader (0) or folded dense (1)
-// line algorithm, and the workgroup size.
+// line algorithm, workgroup size, and the pipelined persistent form (see fused_small.hip).
 #include "kernels.hpp"
 
 namespace fx {
@@ -11,13 +11,22 @@ namespace fx {
 #define FX_17_BOTH(T, TAG, VPW, NT, ALG, ATAG)                                        \
     FX_17(T, -1, VPW, NT, ALG, "fused_17x17x17_" TAG "_fwd_v" #VPW "t" #NT "_" ATAG); \
     FX_17(T, +1, VPW, NT, ALG, "fused_17x17x17_" TAG "_bwd_v" #VPW "t" #NT "_" ATAG)
+#define FX_17P(T, S, VPW, NT, ALG, P, NAME) \
+    r.push_back(fpipe_desc<FusedCfg<17, 17, 17, T, S, VPW, NT, 0, 0, ALG>, P>(NAME))
+#define FX_17P_BOTH(T, TAG, VPW, NT, ALG, ATAG, P)                                                \
+    FX_17P(T, -1, VPW, NT, ALG, P, "pipe_17x17x17_" TAG "_fwd_v" #VPW "t" #NT "p" #P "_" ATAG); \
+    FX_17P(T, +1, VPW, NT, ALG, P, "pipe_17x17x17_" TAG "_bwd_v" #VPW "t" #NT "p" #P "_" ATAG)
 
 void register_fused_17(std::vector<KernelDesc>& r)
 {
-    FX_17_BOTH(double, "c128", 1, 320, 0, "rader");
+    FX_17P_BOTH(double, "c128", 1, 320, 1, "fold", 2);
+    FX_17P_BOTH(double, "c128", 1, 320, 1, "fold", 1);
+    FX_17P_BOTH(double, "c128", 1, 320, 1, "fold", 0);
     FX_17_BOTH(double, "c128", 1, 320, 1, "fold");
+    FX_17_BOTH(double, "c128", 1, 320, 0, "rader");
     FX_17_BOTH(double, "c128", 1, 192, 1, "fold");
-    FX_17_BOTH(float, "c64", 2, 320, 0, "rader");
+    FX_17P_BOTH(float, "c64", 1, 320, 1, "fold", 4);
+    FX_17P_BOTH(float, "c64", 2, 640, 1, "fold", 2);
     FX_17_BOTH(float, "c64", 2, 320, 1, "fold");
     FX_17_BOTH(float, "c64", 1, 320, 1, "fold");
 }

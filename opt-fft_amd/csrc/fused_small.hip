@@ -1,5 +1,8 @@
 // LDS-resident fused kernels for the small geometries: L = 6, 8, 12 and 8x8x16, both
-// precisions and both directions, at two or three tilings each for the create-time race.
+// precisions and both directions, at several tilings each for the create-time race.
+// "pipe_pP" variants are the persistent software-pipelined form (at most P workgroups per
+// CU); "pipe_p0" is the same pass structure with one block per workgroup; "fused_" is the
+// staged form (whole block read into LDS first).
 #include "kernels.hpp"
 
 namespace fx {
@@ -9,28 +12,48 @@ namespace fx {
 #define FX_F_BOTH(N0, N1, N2, T, TAG, VPW, NT, PZ)                                         \
     FX_F(N0, N1, N2, T, -1, VPW, NT, PZ, "fused_" #N0 "x" #N1 "x" #N2 "_" TAG "_fwd_v" #VPW "t" #NT); \
     FX_F(N0, N1, N2, T, +1, VPW, NT, PZ, "fused_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_v" #VPW "t" #NT)
+#define FX_P(N0, N1, N2, T, S, VPW, NT, PZ, P, NAME) \
+    r.push_back(fpipe_desc<FusedCfg<N0, N1, N2, T, S, VPW, NT, PZ, 0, 0>, P>(NAME))
+#define FX_P_BOTH(N0, N1, N2, T, TAG, VPW, NT, PZ, P)                                        \
+    FX_P(N0, N1, N2, T, -1, VPW, NT, PZ, P,                                                 \
+         "pipe_" #N0 "x" #N1 "x" #N2 "_" TAG "_fwd_v" #VPW "t" #NT "p" #P);                  \
+    FX_P(N0, N1, N2, T, +1, VPW, NT, PZ, P,                                                 \
+         "pipe_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_v" #VPW "t" #NT "p" #P)
 
 void register_fused_small(std::vector<KernelDesc>& r)
 {
     // L = 6: 36 lines per volume per pass.
+    FX_P_BOTH(6, 6, 6, double, "c128", 16, 576, 1, 4);
+    FX_P_BOTH(6, 6, 6, double, "c128", 16, 576, 1, 0);
+    FX_P_BOTH(6, 6, 6, double, "c128", 32, 576, 1, 4);
     FX_F_BOTH(6, 6, 6, double, "c128", 16, 576, 1);
-    FX_F_BOTH(6, 6, 6, double, "c128", 16, 192, 1);
+    FX_P_BOTH(6, 6, 6, float, "c64", 32, 576, 1, 4);
+    FX_P_BOTH(6, 6, 6, float, "c64", 16, 576, 1, 0);
     FX_F_BOTH(6, 6, 6, float, "c64", 16, 576, 1);
-    FX_F_BOTH(6, 6, 6, float, "c64", 32, 576, 1);
     // L = 8: 64 lines per volume per pass.
+    FX_P_BOTH(8, 8, 8, double, "c128", 4, 256, 1, 8);
+    FX_P_BOTH(8, 8, 8, double, "c128", 4, 256, 1, 0);
+    FX_P_BOTH(8, 8, 8, double, "c128", 1, 64, 1, 16);
+    FX_P_BOTH(8, 8, 8, double, "c128", 8, 512, 1, 4);
     FX_F_BOTH(8, 8, 8, double, "c128", 1, 64, 1);
     FX_F_BOTH(8, 8, 8, double, "c128", 4, 256, 1);
-    FX_F_BOTH(8, 8, 8, double, "c128", 8, 256, 1);
+    FX_P_BOTH(8, 8, 8, float, "c64", 4, 256, 1, 8);
+    FX_P_BOTH(8, 8, 8, float, "c64", 8, 512, 1, 4);
     FX_F_BOTH(8, 8, 8, float, "c64", 4, 256, 1);
-    FX_F_BOTH(8, 8, 8, float, "c64", 8, 512, 1);
     // L = 12: 144 lines per volume per pass.
+    FX_P_BOTH(12, 12, 12, double, "c128", 4, 576, 1, 2);
+    FX_P_BOTH(12, 12, 12, double, "c128", 1, 192, 1, 8);
+    FX_P_BOTH(12, 12, 12, double, "c128", 4, 576, 1, 0);
     FX_F_BOTH(12, 12, 12, double, "c128", 1, 192, 1);
     FX_F_BOTH(12, 12, 12, double, "c128", 4, 576, 1);
+    FX_P_BOTH(12, 12, 12, float, "c64", 4, 576, 1, 4);
+    FX_P_BOTH(12, 12, 12, float, "c64", 2, 320, 1, 4);
     FX_F_BOTH(12, 12, 12, float, "c64", 4, 576, 1);
-    FX_F_BOTH(12, 12, 12, float, "c64", 2, 320, 1);
-    // 8 x 8 x 16: z lines 64, y and x lines 128 per volume.
+    // 8 x 8 x 16: x lines 128 per volume.
+    FX_P_BOTH(8, 8, 16, double, "c128", 2, 256, 1, 4);
+    FX_P_BOTH(8, 8, 16, double, "c128", 4, 512, 1, 2);
     FX_F_BOTH(8, 8, 16, double, "c128", 2, 256, 1);
-    FX_F_BOTH(8, 8, 16, double, "c128", 4, 512, 1);
+    FX_P_BOTH(8, 8, 16, float, "c64", 4, 512, 1, 4);
     FX_F_BOTH(8, 8, 16, float, "c64", 4, 512, 1);
 }
 

@@ -62,10 +62,21 @@ FX_HD void lds_pass(int tid, cx<T>* sm, int nlines, BaseFn base, const Bank<T>* 
         cx<T> t[N];
 #pragma unroll
         for (int j = 0; j < N; ++j) t[j] = sm[b + j * STRIDE];
-        dft<N, S, T, V17>(t, bk);
-#pragma unroll
-        for (int j = 0; j < N; ++j) sm[b + j * STRIDE] = t[j];
+        dft_put<N, S, T, V17>(t, bk, [&](int k, cx<T> v) { sm[b + k * STRIDE] = v; });
     }
+}
+
+// Waves per SIMD a kernel must sustain so that its LDS-limited number of workgroups fits
+// on a CU (160 KB of LDS, 4 SIMDs, at most 8 waves each): the VGPR budget follows from it.
+constexpr int waves_per_eu(int lds_bytes, int nt, int cap_wgs)
+{
+    const int waves = (nt + 63) / 64;
+    int wgs = lds_bytes > 0 ? 163840 / lds_bytes : 32;
+    if (wgs > 32 / waves) wgs = 32 / waves;
+    if (cap_wgs > 0 && wgs > cap_wgs) wgs = cap_wgs;
+    if (wgs < 1) wgs = 1;
+    const int w = (wgs * waves + 3) / 4;
+    return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
 
 // ------------------------------------------------------------------------------- fused
@@ -81,6 +92,7 @@ struct FusedCfg {
     static constexpr int XS = N1 * YS + PY;      // LDS pitch between x planes
     static constexpr int VS = N0 * XS;           // LDS pitch between volumes
     static constexpr int LDS = VPW * VS;         // elements
+    static constexpr int WPE = waves_per_eu(LDS * (int)sizeof(cx<T>), NT, 0);
 };
 
 template <class C>
@@ -171,7 +183,7 @@ struct Fused {
 };
 
 template <class C>
-__global__ void __launch_bounds__(C::NT)
+__global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE, 8)))
     k_fused(const cx<typename C::T>* __restrict__ in, cx<typename C::T>* __restrict__ out,
             long nvol, const Bank<typename C::T>* __restrict__ bk, typename C::T scl)
 {
@@ -188,18 +200,133 @@ __global__ void __launch_bounds__(C::NT)
     K::xpass(tid, bid, nvol, sm, out, bk, scl);
 }
 
+// ---------------------------------------------------------------------- fused, pipelined
+// Persistent, software-pipelined form of the fused kernel.  A workgroup walks blocks
+// b, b + grid, ...; for each block:
+//   x pass: the x-lines (v, y, z) come straight from HBM into VGPRs -- lanes hold
+//           consecutive (y, z), so each load instruction reads one contiguous run -- are
+//           transformed and written to LDS;
+//   prefetch: the next block's x-lines are issued into the same VGPRs at once, and stay in
+//           flight through the barriers (which wait on LDS only) and the two passes below;
+//   z pass: LDS -> LDS;
+//   y pass: LDS -> HBM; lanes hold consecutive z, so the stores are contiguous runs.
+// Four LDS operations per element instead of six, and HBM reads overlap the arithmetic.
+template <class C>
+struct FusedPipe {
+    using T = typename C::T;
+    static constexpr int NLX = C::VPW * C::PLANE;          // x-lines per block
+    static constexpr int LPT = (NLX + C::NT - 1) / C::NT;  // x-lines per thread
+    using Regs = cx<T>[LPT][C::N0];
+
+    static FX_HD int nvols(long bid, long nvol) { return Fused<C>::nvols(bid, nvol); }
+
+    // Issue the loads of block b's x-lines.  Lanes past the last line re-read a valid
+    // element (their values are never used).
+    static FX_HD void load_x(int tid, long b, long nvol, const cx<T>* __restrict__ in, Regs& R)
+    {
+        const int nl = nvols(b, nvol) * C::PLANE;
+        const cx<T>* __restrict__ src = in + b * C::VPW * (long)C::V;
+#pragma unroll
+        for (int i = 0; i < LPT; ++i) {
+            int l = tid + i * C::NT;
+            l = l < nl ? l : nl - 1;
+            const int v = l / C::PLANE;
+            const int q = l - v * C::PLANE;
+            const cx<T>* __restrict__ p = src + (long)v * C::V + q;
+#pragma unroll
+            for (int j = 0; j < C::N0; ++j) R[i][j] = p[j * C::PLANE];
+        }
+    }
+    // x DFT in registers, results to their LDS slots.
+    static FX_HD void xpass(int tid, long b, long nvol, Regs& R, cx<T>* sm,
+                            const Bank<T>* __restrict__ bk)
+    {
+        const int nl = nvols(b, nvol) * C::PLANE;
+#pragma unroll
+        for (int i = 0; i < LPT; ++i) {
+            const int l = tid + i * C::NT;
+            if (l < nl) {
+                const int v = l / C::PLANE;
+                const int q = l - v * C::PLANE;
+                const int y = q / C::N2;
+                const int z = q - y * C::N2;
+                const int s0 = v * C::VS + y * C::YS + z;
+                dft_put<C::N0, C::S, T, C::V17>(R[i], bk,
+                                                [&](int k, cx<T> w) { sm[s0 + k * C::XS] = w; });
+            }
+        }
+    }
+    static FX_HD void zpass(int tid, long b, long nvol, cx<T>* sm, const Bank<T>* __restrict__ bk)
+    {
+        Fused<C>::zpass(tid, b, nvol, sm, bk);
+    }
+    // y lines (v, x, z): LDS -> HBM.
+    static FX_HD void ypass(int tid, long b, long nvol, const cx<T>* sm, cx<T>* __restrict__ out,
+                            const Bank<T>* __restrict__ bk, T scl)
+    {
+        const int nv = nvols(b, nvol);
+        cx<T>* __restrict__ dst = out + b * C::VPW * (long)C::V;
+        const bool scaled = scl != T(1);
+        for (int l = tid; l < nv * C::N0 * C::N2; l += C::NT) {
+            const int v = l / (C::N0 * C::N2);
+            const int r = l - v * (C::N0 * C::N2);
+            const int x = r / C::N2;
+            const int z = r - x * C::N2;
+            const int s0 = v * C::VS + x * C::XS + z;
+            cx<T> t[C::N1];
+#pragma unroll
+            for (int j = 0; j < C::N1; ++j) t[j] = sm[s0 + j * C::YS];
+            cx<T>* __restrict__ o = dst + (long)v * C::V + x * C::PLANE + z;
+            if (scaled)
+                dft_put<C::N1, C::S, T, C::V17>(t, bk, [&](int k, cx<T> w) { o[k * C::N2] = scale(w, scl); });
+            else
+                dft_put<C::N1, C::S, T, C::V17>(t, bk, [&](int k, cx<T> w) { o[k * C::N2] = w; });
+        }
+    }
+};
+
+template <class C>
+__global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE, 8)))
+    k_fpipe(const cx<typename C::T>* __restrict__ in, cx<typename C::T>* __restrict__ out,
+            long nvol, long nblocks, const Bank<typename C::T>* __restrict__ bk,
+            typename C::T scl)
+{
+    using K = FusedPipe<C>;
+    __shared__ cx<typename C::T> sm[C::LDS];
+    typename K::Regs R;
+    const int tid = threadIdx.x;
+    long b = blockIdx.x;
+    if (b >= nblocks) return;
+    K::load_x(tid, b, nvol, in, R);
+    for (;;) {
+        K::xpass(tid, b, nvol, R, sm, bk);
+        const long nb = b + gridDim.x;
+        if (nb < nblocks) K::load_x(tid, nb, nvol, in, R);
+        __syncthreads();
+        K::zpass(tid, b, nvol, sm, bk);
+        __syncthreads();
+        K::ypass(tid, b, nvol, sm, out, bk, scl);
+        if (nb >= nblocks) break;
+        __syncthreads();
+        b = nb;
+    }
+}
+
 // ------------------------------------------------------------------------------ pencil
+// PIPE = 0: pass A is k_slab (one slab group per workgroup); PIPE = P > 0: pass A is the
+// persistent pipelined k_spipe with at most P resident workgroups per CU.
 template <int N0_, int N1_, int N2_, typename T_, int S_, int SPW_, int NT_, int PZ_,
-          int NTB_, int CHUNK_>
+          int NTB_, int CHUNK_, int PIPE_ = 0>
 struct PencilCfg {
     using T = T_;
     static constexpr int N0 = N0_, N1 = N1_, N2 = N2_, S = S_, SPW = SPW_, NT = NT_;
-    static constexpr int PZ = PZ_, NTB = NTB_, CHUNK = CHUNK_;
+    static constexpr int PZ = PZ_, NTB = NTB_, CHUNK = CHUNK_, PIPE = PIPE_;
     static constexpr int V = N0 * N1 * N2;
     static constexpr int PLANE = N1 * N2;
     static constexpr int YS = N2 + PZ;
     static constexpr int SS = N1 * YS;
     static constexpr int LDS = SPW * SS;
+    static constexpr int WPE = waves_per_eu(LDS * (int)sizeof(cx<T>), NT, PIPE);
 };
 
 template <class C>
@@ -283,7 +410,7 @@ struct Pencil {
 };
 
 template <class C>
-__global__ void __launch_bounds__(C::NT)
+__global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE, 8)))
     k_slab(const cx<typename C::T>* __restrict__ in, cx<typename C::T>* __restrict__ out,
            long nslab, const Bank<typename C::T>* __restrict__ bk)
 {
@@ -296,6 +423,88 @@ __global__ void __launch_bounds__(C::NT)
     K::zpass(tid, bid, nslab, sm, bk);
     __syncthreads();
     K::ypass(tid, bid, nslab, sm, out, bk);
+}
+
+// Pass A, persistent and pipelined: the y-lines (s, z) of a slab group come straight from
+// HBM into VGPRs (lanes hold consecutive z: contiguous runs), are transformed and written
+// to LDS, and the next group's y-lines are issued at once; then z in LDS and a coalesced
+// store of the group.
+template <class C>
+struct SlabPipe {
+    using T = typename C::T;
+    static constexpr int NLY = C::SPW * C::N2;             // y-lines per group
+    static constexpr int LPT = (NLY + C::NT - 1) / C::NT;
+    using Regs = cx<T>[LPT][C::N1];
+
+    static FX_HD int nslabs(long bid, long nslab) { return Pencil<C>::nslabs(bid, nslab); }
+
+    static FX_HD void load_y(int tid, long b, long nslab, const cx<T>* __restrict__ in, Regs& R)
+    {
+        const int nl = nslabs(b, nslab) * C::N2;
+        const cx<T>* __restrict__ src = in + b * C::SPW * (long)C::PLANE;
+#pragma unroll
+        for (int i = 0; i < LPT; ++i) {
+            int l = tid + i * C::NT;
+            l = l < nl ? l : nl - 1;
+            const int s = l / C::N2;
+            const int z = l - s * C::N2;
+            const cx<T>* __restrict__ p = src + (long)s * C::PLANE + z;
+#pragma unroll
+            for (int j = 0; j < C::N1; ++j) R[i][j] = p[j * C::N2];
+        }
+    }
+    static FX_HD void ypass(int tid, long b, long nslab, Regs& R, cx<T>* sm,
+                            const Bank<T>* __restrict__ bk)
+    {
+        const int nl = nslabs(b, nslab) * C::N2;
+#pragma unroll
+        for (int i = 0; i < LPT; ++i) {
+            const int l = tid + i * C::NT;
+            if (l < nl) {
+                dft<C::N1, C::S, T>(R[i], bk);
+                const int s = l / C::N2;
+                const int z = l - s * C::N2;
+#pragma unroll
+                for (int k = 0; k < C::N1; ++k) sm[s * C::SS + k * C::YS + z] = R[i][k];
+            }
+        }
+    }
+    static FX_HD void zpass(int tid, long b, long nslab, cx<T>* sm, const Bank<T>* __restrict__ bk)
+    {
+        Pencil<C>::zpass(tid, b, nslab, sm, bk);
+    }
+    static FX_HD void store(int tid, long b, long nslab, const cx<T>* sm, cx<T>* __restrict__ out)
+    {
+        const int n = nslabs(b, nslab) * C::PLANE;
+        cx<T>* __restrict__ dst = out + b * C::SPW * (long)C::PLANE;
+        for (int e = tid; e < n; e += C::NT) dst[e] = sm[Pencil<C>::slot(e)];
+    }
+};
+
+template <class C>
+__global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE, 8)))
+    k_spipe(const cx<typename C::T>* __restrict__ in, cx<typename C::T>* __restrict__ out,
+            long nslab, long nblocks, const Bank<typename C::T>* __restrict__ bk)
+{
+    using K = SlabPipe<C>;
+    __shared__ cx<typename C::T> sm[C::LDS];
+    typename K::Regs R;
+    const int tid = threadIdx.x;
+    long b = blockIdx.x;
+    if (b >= nblocks) return;
+    K::load_y(tid, b, nslab, in, R);
+    for (;;) {
+        K::ypass(tid, b, nslab, R, sm, bk);
+        const long nb = b + gridDim.x;
+        if (nb < nblocks) K::load_y(tid, nb, nslab, in, R);
+        __syncthreads();
+        K::zpass(tid, b, nslab, sm, bk);
+        __syncthreads();
+        K::store(tid, b, nslab, sm, out);
+        if (nb >= nblocks) break;
+        __syncthreads();
+        b = nb;
+    }
 }
 
 template <class C>

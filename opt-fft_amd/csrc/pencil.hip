@@ -1,32 +1,41 @@
 // Two-pass pencil kernels for volumes that exceed LDS at complex128 (L = 24, 36 and
-// 24x24x48), plus L = 24 at complex64 where the fused form needs a whole 110 KB workgroup.
+// 24x24x48) and their complex64 counterparts.  Names: s<SPW> slabs per workgroup,
+// t<NT> threads, b<NTB> threads of the x-line pass, c<CHUNK> volumes per launch pair
+// (0 = whole batch), p<P> persistent pipelined pass A with at most P workgroups per CU.
 #include "kernels.hpp"
 
 namespace fx {
 
-#define FX_P(N0, N1, N2, T, S, SPW, NT, PZ, NTB, CH, NAME) \
-    r.push_back(pencil_desc<PencilCfg<N0, N1, N2, T, S, SPW, NT, PZ, NTB, CH>>(NAME))
-#define FX_P_BOTH(N0, N1, N2, T, TAG, SPW, NT, PZ, NTB, CH)                                  \
-    FX_P(N0, N1, N2, T, -1, SPW, NT, PZ, NTB, CH,                                            \
-         "pencil_" #N0 "x" #N1 "x" #N2 "_" TAG "_fwd_s" #SPW "t" #NT "b" #NTB "c" #CH);      \
-    FX_P(N0, N1, N2, T, +1, SPW, NT, PZ, NTB, CH,                                            \
-         "pencil_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_s" #SPW "t" #NT "b" #NTB "c" #CH)
+#define FX_P(N0, N1, N2, T, S, SPW, NT, PZ, NTB, CH, PIPE, NAME) \
+    r.push_back(pencil_desc<PencilCfg<N0, N1, N2, T, S, SPW, NT, PZ, NTB, CH, PIPE>>(NAME))
+#define FX_P_BOTH(N0, N1, N2, T, TAG, SPW, NT, PZ, NTB, CH, PIPE)                                \
+    FX_P(N0, N1, N2, T, -1, SPW, NT, PZ, NTB, CH, PIPE,                                          \
+         "pencil_" #N0 "x" #N1 "x" #N2 "_" TAG "_fwd_s" #SPW "t" #NT "b" #NTB "c" #CH "p" #PIPE); \
+    FX_P(N0, N1, N2, T, +1, SPW, NT, PZ, NTB, CH, PIPE,                                          \
+         "pencil_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_s" #SPW "t" #NT "b" #NTB "c" #CH "p" #PIPE)
 
 void register_pencil(std::vector<KernelDesc>& r)
 {
     // L = 36: slab 36x36 = 20.7 KB complex128.
-    FX_P_BOTH(36, 36, 36, double, "c128", 3, 128, 1, 128, 0);
-    FX_P_BOTH(36, 36, 36, double, "c128", 7, 256, 1, 128, 0);
-    FX_P_BOTH(36, 36, 36, double, "c128", 3, 128, 1, 128, 32);
-    FX_P_BOTH(36, 36, 36, float, "c64", 7, 256, 1, 128, 0);
+    FX_P_BOTH(36, 36, 36, double, "c128", 3, 128, 1, 128, 0, 2);
+    FX_P_BOTH(36, 36, 36, double, "c128", 2, 128, 1, 128, 0, 3);
+    FX_P_BOTH(36, 36, 36, double, "c128", 7, 256, 1, 128, 0, 1);
+    FX_P_BOTH(36, 36, 36, double, "c128", 3, 128, 1, 128, 0, 0);
+    FX_P_BOTH(36, 36, 36, double, "c128", 7, 256, 1, 128, 0, 0);
+    FX_P_BOTH(36, 36, 36, float, "c64", 7, 256, 1, 128, 0, 1);
+    FX_P_BOTH(36, 36, 36, float, "c64", 3, 128, 1, 128, 0, 4);
     // L = 24: slab 24x24 = 9.2 KB complex128.
-    FX_P_BOTH(24, 24, 24, double, "c128", 8, 192, 1, 128, 0);
-    FX_P_BOTH(24, 24, 24, double, "c128", 16, 384, 1, 128, 0);
-    FX_P_BOTH(24, 24, 24, float, "c64", 16, 384, 1, 128, 0);
+    FX_P_BOTH(24, 24, 24, double, "c128", 8, 192, 1, 128, 0, 2);
+    FX_P_BOTH(24, 24, 24, double, "c128", 8, 192, 1, 128, 0, 0);
+    FX_P_BOTH(24, 24, 24, double, "c128", 16, 384, 1, 128, 0, 0);
+    FX_P_BOTH(24, 24, 24, float, "c64", 16, 384, 1, 128, 0, 2);
+    FX_P_BOTH(24, 24, 24, float, "c64", 16, 384, 1, 128, 0, 0);
     // 24 x 24 x 48: slab 24x48 = 18.4 KB complex128; z lines 48-point, y and x 24-point.
-    FX_P_BOTH(24, 24, 48, double, "c128", 4, 128, 1, 128, 0);
-    FX_P_BOTH(24, 24, 48, double, "c128", 8, 256, 1, 128, 0);
-    FX_P_BOTH(24, 24, 48, float, "c64", 8, 256, 1, 128, 0);
+    FX_P_BOTH(24, 24, 48, double, "c128", 4, 192, 1, 128, 0, 2);
+    FX_P_BOTH(24, 24, 48, double, "c128", 4, 128, 1, 128, 0, 0);
+    FX_P_BOTH(24, 24, 48, double, "c128", 8, 256, 1, 128, 0, 0);
+    FX_P_BOTH(24, 24, 48, float, "c64", 8, 384, 1, 128, 0, 2);
+    FX_P_BOTH(24, 24, 48, float, "c64", 8, 256, 1, 128, 0, 0);
 }
 
 }  // namespace fx

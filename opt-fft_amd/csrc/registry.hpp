@@ -25,6 +25,7 @@ struct KernelDesc {
     int vpw;         // volumes (fused) or slabs (pencil pass A) per workgroup
     int nthreads;    // workgroup size of the main kernel
     int variant;     // line algorithm variant (17: 0 Rader, 1 folded dense)
+    int persist;     // 0: one block per workgroup; P > 0: persistent, pipelined, <= P per CU
     size_t lds_bytes;
     const char* name;
     LaunchFn launch;

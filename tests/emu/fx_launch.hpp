@@ -74,6 +74,75 @@ hipError_t launch_fused(const void* in, void* out, long nvol, const void* bank, 
     return hipSuccess;
 }
 
+// Persistent kernels are emulated with a grid of at most 3 workgroups, so every workgroup
+// walks several blocks; LDS is re-poisoned at each block (a block must not depend on the
+// previous block's LDS contents).
+constexpr long kEmuPersistentGrid = 3;
+
+template <class C, int PERSIST>
+hipError_t launch_fpipe(const void* in, void* out, long nvol, const void* bank, double scl,
+                        hipStream_t)
+{
+    using T = typename C::T;
+    using K = FusedPipe<C>;
+    struct Box { typename K::Regs r; };
+    if (nvol <= 0) return hipSuccess;
+    const long nblocks = (nvol + C::VPW - 1) / C::VPW;
+    const long grid = PERSIST > 0 ? (nblocks < kEmuPersistentGrid ? nblocks : kEmuPersistentGrid)
+                                  : nblocks;
+    const Bank<T>* bk = (const Bank<T>*)bank;
+    const cx<T>* src = (const cx<T>*)in;
+    cx<T>* dst = (cx<T>*)out;
+    EmuLds<T> lds(C::LDS);
+    std::vector<Box> R(C::NT);
+    for (long w = 0; w < grid; ++w) {
+        long b = w;
+        emu_threads(C::NT, [&](int t) { K::load_x(t, b, nvol, src, R[t].r); });
+        for (;;) {
+            lds.poison();
+            cx<T>* sm = lds.data();
+            emu_threads(C::NT, [&](int t) { K::xpass(t, b, nvol, R[t].r, sm, bk); });
+            const long nb = b + grid;
+            if (nb < nblocks) emu_threads(C::NT, [&](int t) { K::load_x(t, nb, nvol, src, R[t].r); });
+            emu_threads(C::NT, [&](int t) { K::zpass(t, b, nvol, sm, bk); });
+            emu_threads(C::NT, [&](int t) { K::ypass(t, b, nvol, sm, dst, bk, (T)scl); });
+            if (!lds.guards_ok() && g_emu_bad_block < 0) g_emu_bad_block = b;
+            if (nb >= nblocks) break;
+            b = nb;
+        }
+    }
+    return hipSuccess;
+}
+
+template <class C>
+void emu_spipe(const cx<typename C::T>* src, cx<typename C::T>* dst, long nslab,
+               const Bank<typename C::T>* bk)
+{
+    using T = typename C::T;
+    using K = SlabPipe<C>;
+    struct Box { typename K::Regs r; };
+    const long nblocks = (nslab + C::SPW - 1) / C::SPW;
+    const long grid = nblocks < kEmuPersistentGrid ? nblocks : kEmuPersistentGrid;
+    EmuLds<T> lds(C::LDS);
+    std::vector<Box> R(C::NT);
+    for (long w = 0; w < grid; ++w) {
+        long b = w;
+        emu_threads(C::NT, [&](int t) { K::load_y(t, b, nslab, src, R[t].r); });
+        for (;;) {
+            lds.poison();
+            cx<T>* sm = lds.data();
+            emu_threads(C::NT, [&](int t) { K::ypass(t, b, nslab, R[t].r, sm, bk); });
+            const long nb = b + grid;
+            if (nb < nblocks) emu_threads(C::NT, [&](int t) { K::load_y(t, nb, nslab, src, R[t].r); });
+            emu_threads(C::NT, [&](int t) { K::zpass(t, b, nslab, sm, bk); });
+            emu_threads(C::NT, [&](int t) { K::store(t, b, nslab, sm, dst); });
+            if (!lds.guards_ok() && g_emu_bad_block < 0) g_emu_bad_block = b;
+            if (nb >= nblocks) break;
+            b = nb;
+        }
+    }
+}
+
 template <class C>
 hipError_t launch_pencil(const void* in, void* out, long nvol, const void* bank, double scl,
                          hipStream_t)
@@ -90,7 +159,8 @@ hipError_t launch_pencil(const void* in, void* out, long nvol, const void* bank,
         cx<T>* dst = (cx<T>*)out + v0 * C::V;
         const long nslab = nv * C::N0;
         const long ga = (nslab + C::SPW - 1) / C::SPW;
-        for (long b = 0; b < ga; ++b) {
+        if constexpr (C::PIPE > 0) emu_spipe<C>(src, dst, nslab, bk);
+        else for (long b = 0; b < ga; ++b) {
             lds.poison();
             cx<T>* sm = lds.data();
             emu_threads(C::NT, [&](int t) { K::stage(t, b, nslab, src, sm); });
@@ -110,15 +180,23 @@ template <class C>
 KernelDesc fused_desc(const char* name)
 {
     return KernelDesc{C::N0, C::N1, C::N2, prec_of<typename C::T>(), C::S, kFused, C::VPW,
-                      C::NT, C::V17, sizeof(cx<typename C::T>) * (size_t)C::LDS, name,
+                      C::NT, C::V17, 0, sizeof(cx<typename C::T>) * (size_t)C::LDS, name,
                       &launch_fused<C>};
+}
+
+template <class C, int PERSIST>
+KernelDesc fpipe_desc(const char* name)
+{
+    return KernelDesc{C::N0, C::N1, C::N2, prec_of<typename C::T>(), C::S, kFused, C::VPW,
+                      C::NT, C::V17, PERSIST, sizeof(cx<typename C::T>) * (size_t)C::LDS, name,
+                      &launch_fpipe<C, PERSIST>};
 }
 
 template <class C>
 KernelDesc pencil_desc(const char* name)
 {
     return KernelDesc{C::N0, C::N1, C::N2, prec_of<typename C::T>(), C::S, kPencil, C::SPW,
-                      C::NT, 0, sizeof(cx<typename C::T>) * (size_t)C::LDS, name,
+                      C::NT, 0, C::PIPE, sizeof(cx<typename C::T>) * (size_t)C::LDS, name,
                       &launch_pencil<C>};
 }
 
