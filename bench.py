@@ -24,6 +24,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of the headline kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE
+# (profiles/), or None until measured.
+TRAFFIC = None
 ROTATE_BYTES = 2 << 30
 
 HEADLINE = dict(name="L17_c128_fwd_b1024", shape=(17, 17, 17), batch=1024, dist="uniform",
@@ -75,11 +78,24 @@ def measure(pkg, wl, steps, warmup, check=True):
     e1.synchronize()
     h.synchronize()
     ms = e0.elapsed_ms(e1)
+    # Per-launch durations (events bracketing each execute on its stream) for the roofline.
+    pairs = [(h.Event(), h.Event()) for _ in range(min(steps, 50))]
+    for i, (a, b) in enumerate(pairs):
+        a.record(stream)
+        plan.execute(ins[i % rot].addr, outs[i % rot].addr, stream=stream)
+        b.record(stream)
+    pairs[-1][1].synchronize()
+    per = [a.elapsed_ms(b) for a, b in pairs]
+    for a, b in pairs:
+        a.close()
+        b.close()
+    launch_ms = sum(per) / len(per)
     res = dict(kernel=plan.kernel, rotate_pairs=rot, steps=steps, ms_total=ms,
-               ms_per_execute=ms / steps, transforms_per_s=batch * steps / (ms * 1e-3))
+               ms_per_execute=ms / steps, transforms_per_s=batch * steps / (ms * 1e-3),
+               launch_us_mean=launch_ms * 1e3, launch_us_min=min(per) * 1e3)
     algo = 2.0 * nbytes
     res["algorithmic_bytes_per_execute"] = algo
-    res["achieved_GBs"] = algo / (ms * 1e-3 / steps) / 1e9
+    res["achieved_GBs"] = algo / (launch_ms * 1e-3) / 1e9
     res["roofline_frac"] = res["achieved_GBs"] / PEAK_HBM_GBS
     if check:
         import oracle
@@ -93,8 +109,46 @@ def measure(pkg, wl, steps, warmup, check=True):
     return res
 
 
+def _ref_exe(kind):
+    isa = "v4" if "avx512f" in open("/proc/cpuinfo").read() else "v3"
+    p = os.path.join(ROOT, "oracle", "_ref", isa, f"ref_bench_{kind}")
+    return p if os.path.exists(p) else None
+
+
 def cpu_baseline(seconds=10.0):
-    """The oracle's numpy.fft path ("port") on the host: single thread, bounded sample."""
+    """The reference's own CPU kernels on this host, on a bounded sample of the headline
+    workload (17^3 complex128): all cores through its OpenMP kernel, plus one pinned core
+    through its best-exemplar library.  Falls back to the oracle's numpy.fft ("port")."""
+    import subprocess
+    mt, st = _ref_exe("mt"), _ref_exe("st")
+    if mt and st:
+        ncores = len(os.sched_getaffinity(0))
+        env = dict(os.environ, OMP_NUM_THREADS=str(ncores), OMP_PROC_BIND="close",
+                   OMP_PLACES="cores")
+        r = subprocess.run([mt, "--L", "17", "--batch", "1024", "--seconds", str(seconds)],
+                           capture_output=True, text=True, env=env, timeout=300)
+        r1 = subprocess.run(["taskset", "-c", "0", st, "--L", "17", "--batch", "256",
+                             "--seconds", str(seconds / 2)], capture_output=True, text=True,
+                            timeout=300)
+        if r.returncode == 0 and r1.returncode == 0:
+            m, s1 = json.loads(r.stdout.strip().splitlines()[-1]), json.loads(r1.stdout.strip().splitlines()[-1])
+            model = ""
+            try:
+                model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                         if l.startswith("model name")][0]
+            except Exception:
+                pass
+            return dict(value=m["transforms_per_s"], unit="transforms/s", cores=m["threads"],
+                        kind="reference",
+                        sample=f"reference {m['kernel']} (bench/mt/impl_4, OpenMP, "
+                               f"{m['threads']} threads) on 17^3 complex128 x 1024, "
+                               f"{m['reps']} executes in {m['seconds']:.1f} s on {model}",
+                        single_core=dict(value=s1["transforms_per_s"], cores=1,
+                                         kernel=s1["kernel"],
+                                         sample=f"reference best-exemplar library "
+                                                f"(bench/geom/best, picked {s1['kernel']}) "
+                                                f"on 17^3 x 256, taskset -c 0, "
+                                                f"{s1['seconds']:.1f} s"))
     import oracle
     x = make_input((17, 17, 17), 64, seed=7, dist="uniform")
     n, t0 = 0, time.perf_counter()
@@ -120,6 +174,10 @@ def main():
               "shard across GPUs with no communication (see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    # The CPU baseline spawns the reference binaries, so it runs before HIP is initialised.
+    cpu = None if a.no_cpu else cpu_baseline()
+    if cpu:
+        print(f"# cpu_baseline: {cpu}", file=sys.stderr, flush=True)
     import __graft_entry__
     pkg = __graft_entry__.load_package()
     if pkg.hip.device_count() < 1:
@@ -134,7 +192,6 @@ def main():
         for wl in SECONDARY:
             secondary[wl["name"]] = measure(pkg, wl, max(10, a.steps // 2), a.warmup)
             print(f"# {wl['name']}: {secondary[wl['name']]}", file=sys.stderr, flush=True)
-    cpu = None if a.no_cpu else cpu_baseline()
     V = 17 ** 3
     line = {
         "metric": "batched transforms/s + % HBM roofline at L=8/17/36 fp64; serving p99 latency",
@@ -157,7 +214,9 @@ def main():
         "gflops_nominal": 5.0 * V * math.log2(V) * head["transforms_per_s"] / 1e9,
         "rel_l2_vs_numpy": head["rel_l2_vs_numpy"],
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": head["roofline_frac"], "traffic": None},
+                     "unit": "GB/s", "frac": head["roofline_frac"], "traffic": TRAFFIC,
+                     "kernel_us_mean": head["launch_us_mean"],
+                     "algorithmic_bytes_per_launch": head["algorithmic_bytes_per_execute"]},
         "cpu_baseline": cpu,
         "secondary": secondary,
     }
