@@ -76,6 +76,16 @@ int fft3d_mx_execute(fft3d_mx_plan *plan, const void *in, void *out, void *strea
 int fft3d_mx_execute_n(fft3d_mx_plan *plan, const void *in, void *out, long batch,
                        void *stream);
 
+/* Chain of m transforms, each output multiplied by step_scale and fed to the next step
+ * (the reference driver's --chain M mode, bench/geom/driver.c:147-164; step_scale =
+ * 1/sqrt(V) is its --unitary mode, whose end state has the closed form FFT^2 = V * index
+ * reversal, bench/geom/check.py:43-61).  The scaling is fused into each step's kernel.
+ * Steps ping-pong between `out` and `pong` (both device buffers of the plan's batch size,
+ * distinct from each other and from `in`) so that the end state always lands in `out`.
+ * Enqueued on `stream`; m >= 1.  0 or a negative error code. */
+int fft3d_mx_execute_chain(fft3d_mx_plan *plan, const void *in, void *out, void *pong, int m,
+                           double step_scale, void *stream);
+
 void fft3d_mx_destroy(fft3d_mx_plan *plan);
 
 /* Introspection of a live plan. */

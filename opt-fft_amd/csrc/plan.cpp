@@ -335,6 +335,34 @@ extern "C" int fft3d_mx_execute(fft3d_mx_plan* p, const void* in, void* out, voi
     return fft3d_mx_execute_n(p, in, out, p->batch, stream);
 }
 
+extern "C" int fft3d_mx_execute_chain(fft3d_mx_plan* p, const void* in, void* out, void* pong,
+                                      int m, double step_scale, void* stream)
+{
+    if (!p) return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute_chain: NULL plan");
+    if (m < 1) return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute_chain: m = %d < 1", m);
+    if (!in || !out || !pong || in == (const void*)out || in == (const void*)pong || out == pong)
+        return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute_chain: in, out, pong must be distinct");
+    if (!(step_scale == step_scale))
+        return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute_chain: step_scale is NaN");
+    int cur = p->device;
+    (void)hipGetDevice(&cur);
+    if (cur != p->device) (void)hipSetDevice(p->device);
+    // With an even m the first step writes pong, so the m-th lands in out.
+    const void* src = in;
+    void* dst = (m % 2) ? out : pong;
+    hipError_t e = hipSuccess;
+    for (int s = 0; s < m && e == hipSuccess; ++s) {
+        e = p->k->launch(src, dst, p->batch, p->bank, step_scale, (hipStream_t)stream);
+        src = dst;
+        dst = dst == out ? pong : out;
+    }
+    if (cur != p->device) (void)hipSetDevice(cur);
+    if (e != hipSuccess)
+        return fail(FFT3D_MX_EHIP, "fft3d_mx_execute_chain: %s: %s", p->k->name,
+                    hipGetErrorString(e));
+    return FFT3D_MX_OK;
+}
+
 extern "C" void fft3d_mx_destroy(fft3d_mx_plan* p)
 {
     if (!p) return;

@@ -35,7 +35,8 @@ EXPORTED = [
     "fft3d_gpu_name", "fft3d_gpu_description", "fft3d_gpu_supports", "fft3d_gpu_create",
     "fft3d_gpu_execute", "fft3d_gpu_destroy",
     "fft3d_mx_last_error", "fft3d_mx_clear_error", "fft3d_mx_supports", "fft3d_mx_create",
-    "fft3d_mx_create_named", "fft3d_mx_execute", "fft3d_mx_execute_n", "fft3d_mx_destroy",
+    "fft3d_mx_create_named", "fft3d_mx_execute", "fft3d_mx_execute_n",
+    "fft3d_mx_execute_chain", "fft3d_mx_destroy",
     "fft3d_mx_plan_get_info", "fft3d_mx_kernel_count", "fft3d_mx_kernel_name",
     "fft3d_mx_bank_image", "fft3d_mx_volume_bytes",
 ]
@@ -85,6 +86,7 @@ def load():
         L.fft3d_mx_create_named.restype = vp
         L.fft3d_mx_execute.argtypes = [vp, vp, vp, vp]
         L.fft3d_mx_execute_n.argtypes = [vp, vp, vp, cl, vp]
+        L.fft3d_mx_execute_chain.argtypes = [vp, vp, vp, vp, ci, ctypes.c_double, vp]
         L.fft3d_mx_destroy.argtypes = [vp]
         L.fft3d_mx_plan_get_info.argtypes = [vp, ctypes.POINTER(PlanInfo)]
         L.fft3d_mx_kernel_name.argtypes = [ci, ip, ip, ip, ip, ip]
@@ -174,6 +176,14 @@ class Plan:
         s = stream.handle if isinstance(stream, hip.Stream) else stream
         n = self.batch if batch is None else int(batch)
         rc = load().fft3d_mx_execute_n(self.handle, din, dout, n, s)
+        if rc != 0:
+            raise RuntimeError(last_error())
+
+    def chain(self, din, dout, dpong, m, step_scale, stream=None):
+        """m chained transforms (each scaled by step_scale); the end state lands in dout."""
+        s = stream.handle if isinstance(stream, hip.Stream) else stream
+        rc = load().fft3d_mx_execute_chain(self.handle, din, dout, dpong, int(m),
+                                           float(step_scale), s)
         if rc != 0:
             raise RuntimeError(last_error())
 

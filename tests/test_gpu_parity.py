@@ -175,3 +175,31 @@ def test_mirror_fftn_ifftn(gpu):
     # a leading batch of more than one axis rides along
     x5 = oracle.make_input((8, 8, 8), 6, seed=2).reshape(2, 3, 8, 8, 8)
     assert oracle.rel_l2(f.fftn(x5), np.fft.fftn(x5, axes=(-3, -2, -1))) <= 1e-13
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (17, 17, 17), (36, 36, 36)])
+def test_unitary_chain_closed_form(gpu, shape):
+    """--chain M --unitary gate (bench/geom/check.py:43-72): after m unitary steps the state
+    is x, FFT(x)/sqrt(V), x[-j] or FFT(x[-j])/sqrt(V); tolerance 1e-12 * sqrt(m) as there,
+    tightened to 1e-13 * sqrt(m)."""
+    f, h = gpu.fft3d, gpu.hip
+    batch = 3
+    x = oracle.make_input(shape, batch, seed=77)
+    V = shape[0] * shape[1] * shape[2]
+    plan = f.Plan(shape, batch)
+    bufs = [h.DeviceBuffer(x.nbytes) for _ in range(3)]
+    bufs[0].upload(x)
+    for m in (1, 2, 3, 4, 5, 64):
+        plan.chain(bufs[0].addr, bufs[1].addr, bufs[2].addr, m, 1.0 / np.sqrt(V))
+        h.synchronize()
+        y = bufs[1].download(np.complex128, x.shape)
+        err = oracle.rel_l2(y, oracle.chain_expected(x, m))
+        assert err <= 1e-13 * max(1.0, np.sqrt(m)), (m, err)
+        assert np.array_equal(bufs[0].download(np.complex128, x.shape), x)
+    with pytest.raises(RuntimeError):
+        plan.chain(bufs[0].addr, bufs[1].addr, bufs[1].addr, 2, 1.0)
+    with pytest.raises(RuntimeError):
+        plan.chain(bufs[0].addr, bufs[1].addr, bufs[2].addr, 0, 1.0)
+    for b in bufs:
+        b.close()
+    plan.close()
