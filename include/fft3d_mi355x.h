@@ -117,6 +117,30 @@ const char *fft3d_mx_kernel_name(int i, int *n0, int *n1, int *n2, int *precisio
  * Writes at most `cap` bytes to dst (dst may be NULL to query *size).  0 or error. */
 int fft3d_mx_bank_image(int precision, void *dst, size_t cap, size_t *size);
 
+/* ---------------------------------------------------------------------------------------
+ * One persistent launch over a mixed batch.  Each item is a batch of volumes of one of the
+ * fused geometries (6^3, 8^3, 12^3, 17^3, 8x8x16), in either direction; all items share
+ * one precision.  create() cuts the items into blocks, orders them heaviest first, races the
+ * compiled tile presets on the device and keeps the fastest; execute() is ONE kernel launch
+ * that transforms every item (in[i] -> out[i], device buffers of items[i].batch volumes). */
+typedef struct fft3d_mx_item {
+    int n0, n1, n2;
+    int batch;
+    int direction; /* FFT3D_MX_FORWARD or FFT3D_MX_BACKWARD */
+} fft3d_mx_item;
+typedef struct fft3d_mx_multi fft3d_mx_multi;
+
+/* At most 16 items.  flags: FFT3D_MX_NORMALIZE (1/V on every item), FFT3D_MX_NO_TUNE.
+ * $FFT3D_MX_MULTI_KERNEL=<name> forces a preset.  NULL on failure. */
+fft3d_mx_multi *fft3d_mx_multi_create(const fft3d_mx_item *items, int nitems, int precision,
+                                      unsigned flags);
+int fft3d_mx_multi_execute(fft3d_mx_multi *m, const void *const *in, void *const *out,
+                           void *stream);
+/* Kernel name of the plan (at most cap-1 chars written), task count and grid size. */
+int fft3d_mx_multi_get_info(const fft3d_mx_multi *m, char *kernel, size_t cap, int *ntasks,
+                            int *grid);
+void fft3d_mx_multi_destroy(fft3d_mx_multi *m);
+
 /* Bytes of device memory one volume of this geometry/precision occupies. */
 size_t fft3d_mx_volume_bytes(int n0, int n1, int n2, int precision);
 

@@ -1,0 +1,27 @@
+// Device launcher and registry descriptor for the mixed-batch kernel (multi.hpp).
+#pragma once
+
+namespace fx {
+
+template <typename T, int PRESET>
+hipError_t launch_multi(const void* args, const int2* tasks, int ntasks, const void* bank,
+                        hipStream_t stream)
+{
+    using M = MultiCfg<T, PRESET>;
+    if (ntasks <= 0) return hipSuccess;
+    const long grid = persistent_grid(k_multi<T, PRESET>, M::NT, 8, ntasks);
+    hipLaunchKernelGGL((k_multi<T, PRESET>), dim3((unsigned)grid), dim3(M::NT), 0, stream,
+                       *(const MultiArgs<T>*)args, tasks, ntasks, (const Bank<T>*)bank);
+    return hipGetLastError();
+}
+
+template <typename T, int PRESET>
+MultiDesc multi_desc(const char* name)
+{
+    using M = MultiCfg<T, PRESET>;
+    return MultiDesc{prec_of<T>(), PRESET, M::NT,
+                     {M::vpw(0), M::vpw(1), M::vpw(2), M::vpw(3), M::vpw(4)},
+                     sizeof(cx<T>) * (size_t)M::LDS, name, &launch_multi<T, PRESET>};
+}
+
+}  // namespace fx

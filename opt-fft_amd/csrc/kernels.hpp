@@ -261,8 +261,9 @@ struct FusedPipe {
         Fused<C>::zpass(tid, b, nvol, sm, bk);
     }
     // y lines (v, x, z): LDS -> HBM.
+    // cj: conjugate every output (the backward transform through the forward path).
     static FX_HD void ypass(int tid, long b, long nvol, const cx<T>* sm, cx<T>* __restrict__ out,
-                            const Bank<T>* __restrict__ bk, T scl)
+                            const Bank<T>* __restrict__ bk, T scl, bool cj = false)
     {
         const int nv = nvols(b, nvol);
         cx<T>* __restrict__ dst = out + b * C::VPW * (long)C::V;
@@ -277,7 +278,11 @@ struct FusedPipe {
 #pragma unroll
             for (int j = 0; j < C::N1; ++j) t[j] = sm[s0 + j * C::YS];
             cx<T>* __restrict__ o = dst + (long)v * C::V + x * C::PLANE + z;
-            if (scaled)
+            if (cj)
+                dft_put<C::N1, C::S, T, C::V17>(t, bk, [&](int k, cx<T> w) {
+                    o[k * C::N2] = cx<T>{w.r * scl, -(w.i * scl)};
+                });
+            else if (scaled)
                 dft_put<C::N1, C::S, T, C::V17>(t, bk, [&](int k, cx<T> w) { o[k * C::N2] = scale(w, scl); });
             else
                 dft_put<C::N1, C::S, T, C::V17>(t, bk, [&](int k, cx<T> w) { o[k * C::N2] = w; });

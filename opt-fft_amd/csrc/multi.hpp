@@ -1,0 +1,153 @@
+// One persistent launch over a mixed batch: several items, each a batch of volumes of one
+// of the fused geometries (6^3, 8^3, 12^3, 17^3, 8x8x16) in either direction.
+//
+// The host cuts every item into blocks (VPW volumes, the tile the preset gives that
+// geometry), sorts the blocks heaviest first into a task list, and launches one grid of
+// resident workgroups that walk the list round-robin.  A task runs the pipelined fused
+// pass structure of its geometry (x from HBM into VGPRs, z in LDS, y from LDS to HBM) out
+// of one LDS buffer sized for the largest block.  The backward direction reuses the forward
+// code: conj(DFT(conj(x))) is the unnormalised backward DFT, and both conjugations are sign
+// flips folded into the load and the store.
+#pragma once
+#include "kernels.hpp"
+
+namespace fx {
+
+constexpr int kMultiMaxItems = 16;
+constexpr int kMultiGeoms = 5;   // 6^3, 8^3, 12^3, 17^3, 8x8x16
+constexpr int kMultiShape[kMultiGeoms][3] = {{6, 6, 6}, {8, 8, 8}, {12, 12, 12}, {17, 17, 17},
+                                             {8, 8, 16}};
+
+template <typename T>
+struct MultiItem {
+    const cx<T>* in;
+    cx<T>* out;
+    long nvol;
+    int code;    // index into kMultiShape
+    int conj;    // 1 for the backward direction
+    T scale;
+};
+
+template <typename T>
+struct MultiArgs {
+    MultiItem<T> item[kMultiMaxItems];
+};
+
+// Tile presets: workgroup size and volumes per block for each geometry code.
+template <typename T, int PRESET>
+struct MultiPreset;
+// 320 threads: 90-100% of the lanes busy in every line pass.
+template <typename T>
+struct MultiPreset<T, 0> {
+    static constexpr int NT = 320;
+    using G0 = FusedCfg<6, 6, 6, T, -1, 8, NT, 1, 0, 1>;
+    using G1 = FusedCfg<8, 8, 8, T, -1, 5, NT, 1, 0, 1>;
+    using G2 = FusedCfg<12, 12, 12, T, -1, 2, NT, 1, 0, 1>;
+    using G3 = FusedCfg<17, 17, 17, T, -1, 1, NT, 0, 0, 1>;
+    using G4 = FusedCfg<8, 8, 16, T, -1, 2, NT, 1, 0, 1>;
+};
+// 640 threads: twice the volumes per block.
+template <typename T>
+struct MultiPreset<T, 1> {
+    static constexpr int NT = 640;
+    using G0 = FusedCfg<6, 6, 6, T, -1, 16, NT, 1, 0, 1>;
+    using G1 = FusedCfg<8, 8, 8, T, -1, 10, NT, 1, 0, 1>;
+    using G2 = FusedCfg<12, 12, 12, T, -1, 4, NT, 1, 0, 1>;
+    using G3 = FusedCfg<17, 17, 17, T, -1, 2, NT, 0, 0, 1>;
+    using G4 = FusedCfg<8, 8, 16, T, -1, 5, NT, 1, 0, 1>;
+};
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+template <typename T, int PRESET>
+struct MultiCfg {
+    using P = MultiPreset<T, PRESET>;
+    static constexpr int NT = P::NT;
+    static constexpr int LDS =
+        cmax(cmax(cmax(P::G0::LDS, P::G1::LDS), cmax(P::G2::LDS, P::G3::LDS)), P::G4::LDS);
+    static constexpr int WPE = waves_per_eu(LDS * (int)sizeof(cx<T>), NT, 0);
+    static constexpr int vpw(int code)
+    {
+        return code == 0 ? P::G0::VPW : code == 1 ? P::G1::VPW : code == 2 ? P::G2::VPW
+             : code == 3 ? P::G3::VPW : P::G4::VPW;
+    }
+};
+
+// Phase `ph` (0: x from HBM, 1: z, 2: y to HBM) of one block of one item.  Phases are
+// separated by workgroup barriers; no register state crosses a barrier.
+template <class C>
+FX_HD void multi_phase(int ph, int tid, long b, const MultiItem<typename C::T>& it,
+                       cx<typename C::T>* sm, const Bank<typename C::T>* __restrict__ bk)
+{
+    using K = FusedPipe<C>;
+    if (ph == 0) {
+        typename K::Regs R;
+        K::load_x(tid, b, it.nvol, it.in, R);
+        if (it.conj) {
+#pragma unroll
+            for (int i = 0; i < K::LPT; ++i)
+#pragma unroll
+                for (int j = 0; j < C::N0; ++j) R[i][j].i = -R[i][j].i;
+        }
+        K::xpass(tid, b, it.nvol, R, sm, bk);
+    } else if (ph == 1) {
+        K::zpass(tid, b, it.nvol, sm, bk);
+    } else {
+        K::ypass(tid, b, it.nvol, sm, it.out, bk, it.scale, it.conj != 0);
+    }
+}
+
+template <typename T, int PRESET>
+FX_HD void multi_task_phase(int ph, int tid, int2 task, const MultiArgs<T>& a, cx<T>* sm,
+                            const Bank<T>* __restrict__ bk)
+{
+    using P = MultiPreset<T, PRESET>;
+    const MultiItem<T>& it = a.item[task.x];
+    switch (it.code) {
+    case 0: multi_phase<typename P::G0>(ph, tid, task.y, it, sm, bk); break;
+    case 1: multi_phase<typename P::G1>(ph, tid, task.y, it, sm, bk); break;
+    case 2: multi_phase<typename P::G2>(ph, tid, task.y, it, sm, bk); break;
+    case 3: multi_phase<typename P::G3>(ph, tid, task.y, it, sm, bk); break;
+    default: multi_phase<typename P::G4>(ph, tid, task.y, it, sm, bk); break;
+    }
+}
+
+template <typename T, int PRESET>
+__global__ void __launch_bounds__((MultiCfg<T, PRESET>::NT))
+    __attribute__((amdgpu_waves_per_eu(MultiCfg<T, PRESET>::WPE, 8)))
+    k_multi(const MultiArgs<T> a, const int2* __restrict__ tasks, int ntasks,
+            const Bank<T>* __restrict__ bk)
+{
+    using M = MultiCfg<T, PRESET>;
+    __shared__ cx<T> sm[M::LDS];
+    const int tid = threadIdx.x;
+    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const int2 task = tasks[t];
+        multi_task_phase<T, PRESET>(0, tid, task, a, sm, bk);
+        __syncthreads();
+        multi_task_phase<T, PRESET>(1, tid, task, a, sm, bk);
+        __syncthreads();
+        multi_task_phase<T, PRESET>(2, tid, task, a, sm, bk);
+        __syncthreads();
+    }
+}
+
+// Registry entry of a multi kernel.
+typedef hipError_t (*MultiLaunchFn)(const void* args, const int2* tasks, int ntasks,
+                                    const void* bank, hipStream_t stream);
+struct MultiDesc {
+    int prec;
+    int preset;
+    int nthreads;
+    int vpw[kMultiGeoms];
+    size_t lds_bytes;
+    const char* name;
+    MultiLaunchFn launch;
+};
+
+void register_multi(std::vector<MultiDesc>& r);
+const std::vector<MultiDesc>& multi_registry();
+
+}  // namespace fx
+
+#include <fx_multi_launch.hpp>

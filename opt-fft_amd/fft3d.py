@@ -39,7 +39,14 @@ EXPORTED = [
     "fft3d_mx_execute_chain", "fft3d_mx_destroy",
     "fft3d_mx_plan_get_info", "fft3d_mx_kernel_count", "fft3d_mx_kernel_name",
     "fft3d_mx_bank_image", "fft3d_mx_volume_bytes",
+    "fft3d_mx_multi_create", "fft3d_mx_multi_execute", "fft3d_mx_multi_get_info",
+    "fft3d_mx_multi_destroy",
 ]
+
+
+class Item(ctypes.Structure):
+    _fields_ = [("n0", ctypes.c_int), ("n1", ctypes.c_int), ("n2", ctypes.c_int),
+                ("batch", ctypes.c_int), ("direction", ctypes.c_int)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -94,6 +101,11 @@ def load():
         L.fft3d_mx_bank_image.argtypes = [ci, vp, cs, ctypes.POINTER(cs)]
         L.fft3d_mx_volume_bytes.argtypes = [ci, ci, ci, ci]
         L.fft3d_mx_volume_bytes.restype = cs
+        L.fft3d_mx_multi_create.argtypes = [ctypes.POINTER(Item), ci, ci, cu]
+        L.fft3d_mx_multi_create.restype = vp
+        L.fft3d_mx_multi_execute.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), vp]
+        L.fft3d_mx_multi_get_info.argtypes = [vp, ctypes.c_char_p, cs, ip, ip]
+        L.fft3d_mx_multi_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -265,3 +277,64 @@ def ifft3d(x):
     if x.ndim not in (3, 4):
         raise ValueError(f"ifft3d expects 3 or 4 dimensions, got {x.shape}")
     return ifftn(x)
+
+
+class MultiPlan:
+    """One persistent launch over a mixed batch: items = [(shape, batch, direction), ...]."""
+
+    def __init__(self, items, precision=C128, flags=0):
+        L = load()
+        self.items = [(tuple(int(n) for n in sh), int(b), int(d)) for sh, b, d in items]
+        arr = (Item * len(self.items))(*[Item(*sh, b, d) for sh, b, d in self.items])
+        h = L.fft3d_mx_multi_create(arr, len(self.items), int(precision), int(flags))
+        if not h:
+            raise ValueError(last_error())
+        self.handle = ctypes.c_void_p(h)
+        self.precision = int(precision)
+        self.dtype = np.complex128 if precision == C128 else np.complex64
+
+    def info(self):
+        name = ctypes.create_string_buffer(96)
+        nt, gr = ctypes.c_int(0), ctypes.c_int(0)
+        if load().fft3d_mx_multi_get_info(self.handle, name, 96, ctypes.byref(nt),
+                                          ctypes.byref(gr)) != 0:
+            raise ValueError(last_error())
+        return name.value.decode(), nt.value
+
+    def execute(self, ins, outs, stream=None):
+        n = len(self.items)
+        s = stream.handle if isinstance(stream, hip.Stream) else stream
+        a = (ctypes.c_void_p * n)(*ins)
+        b = (ctypes.c_void_p * n)(*outs)
+        if load().fft3d_mx_multi_execute(self.handle, a, b, s) != 0:
+            raise RuntimeError(last_error())
+
+    def run(self, xs):
+        """Host convenience: upload every item, one launch, download every result."""
+        ins, outs = [], []
+        try:
+            for x in xs:
+                x = np.ascontiguousarray(x, dtype=self.dtype)
+                bi, bo = hip.DeviceBuffer(x.nbytes), hip.DeviceBuffer(x.nbytes)
+                bi.upload(x)
+                ins.append((bi, x.shape))
+                outs.append(bo)
+            self.execute([b.addr for b, _ in ins], [b.addr for b in outs])
+            hip.synchronize()
+            return [o.download(self.dtype, shp) for o, (_, shp) in zip(outs, ins)]
+        finally:
+            for b, _ in ins:
+                b.close()
+            for b in outs:
+                b.close()
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            load().fft3d_mx_multi_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

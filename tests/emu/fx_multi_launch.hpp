@@ -1,0 +1,37 @@
+// TEST INFRASTRUCTURE: host emulation of the mixed-batch launcher (see fx_launch.hpp for the
+// emulation rules).  The grid is 3 workgroups walking the task list round-robin; each task
+// runs its three phases over all threads in barrier order, LDS re-poisoned per task.
+#pragma once
+
+namespace fx {
+
+template <typename T, int PRESET>
+hipError_t launch_multi(const void* args, const int2* tasks, int ntasks, const void* bank,
+                        hipStream_t)
+{
+    using M = MultiCfg<T, PRESET>;
+    const MultiArgs<T>& a = *(const MultiArgs<T>*)args;
+    const Bank<T>* bk = (const Bank<T>*)bank;
+    EmuLds<T> lds(M::LDS);
+    const int grid = ntasks < 3 ? ntasks : 3;
+    for (int w = 0; w < grid; ++w)
+        for (int t = w; t < ntasks; t += grid) {
+            lds.poison();
+            cx<T>* sm = lds.data();
+            for (int ph = 0; ph < 3; ++ph)
+                emu_threads(M::NT, [&](int tid) { multi_task_phase<T, PRESET>(ph, tid, tasks[t], a, sm, bk); });
+            if (!lds.guards_ok() && g_emu_bad_block < 0) g_emu_bad_block = t;
+        }
+    return hipSuccess;
+}
+
+template <typename T, int PRESET>
+MultiDesc multi_desc(const char* name)
+{
+    using M = MultiCfg<T, PRESET>;
+    return MultiDesc{prec_of<T>(), PRESET, M::NT,
+                     {M::vpw(0), M::vpw(1), M::vpw(2), M::vpw(3), M::vpw(4)},
+                     sizeof(cx<T>) * (size_t)M::LDS, name, &launch_multi<T, PRESET>};
+}
+
+}  // namespace fx

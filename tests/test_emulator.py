@@ -75,3 +75,32 @@ def test_asan_sweep():
                        timeout=600)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "ok:" in r.stdout
+
+
+MIX = [((6, 6, 6), 20, -1), ((8, 8, 8), 11, 1), ((12, 12, 12), 3, -1), ((17, 17, 17), 2, 1),
+       ((8, 8, 16), 5, -1), ((8, 8, 8), 1, -1), ((6, 6, 6), 0, 1)]
+
+
+def test_mixed_batch_kernels_on_the_emulator(emu):
+    emu.emu_multi_name.restype = ctypes.c_char_p
+    n = len(MIX)
+    assert emu.emu_multi_count() >= 4
+    for i in range(emu.emu_multi_count()):
+        name = emu.emu_multi_name(i)
+        prec = 0 if b"c128" in name else 1
+        dt = np.complex128 if prec == 0 else np.complex64
+        xs = [oracle.make_input(s, max(b, 1), seed=k, dtype=dt)[:b] for k, (s, b, d) in enumerate(MIX)]
+        ys = [np.full_like(x, np.nan) for x in xs]
+        scale = [1.0 if d < 0 else 0.5 for _, _, d in MIX]
+        rc = emu.emu_multi(name, n, (ctypes.c_int * (3 * n))(*[v for s, _, _ in MIX for v in s]),
+                           (ctypes.c_int * n)(*[b for _, b, _ in MIX]),
+                           (ctypes.c_int * n)(*[d for _, _, d in MIX]),
+                           (ctypes.c_double * n)(*scale),
+                           (ctypes.c_void_p * n)(*[x.ctypes.data for x in xs]),
+                           (ctypes.c_void_p * n)(*[y.ctypes.data for y in ys]))
+        assert rc == 0, (name, rc)
+        for x, y, (s, b, d), sc in zip(xs, ys, MIX, scale):
+            if b == 0:
+                continue
+            err = oracle.rel_l2(y, sc * oracle.fftn_numpy(x, sign=d))
+            assert err <= (1e-13 if prec == 0 else 2e-6), (name, s, d, err)
