@@ -132,9 +132,42 @@ __global__ void __launch_bounds__((MultiCfg<T, PRESET>::NT))
     }
 }
 
+// Same kernel with the items, the task count and the task list read from device memory, so a
+// launch can be captured once in a hipGraph and replayed with new descriptors (serving).
+template <typename T>
+struct MultiDescriptor {
+    MultiArgs<T> args;
+    int ntasks;
+    int pad_;
+    int2 tasks[1];  // ntasks entries follow
+};
+
+template <typename T, int PRESET>
+__global__ void __launch_bounds__((MultiCfg<T, PRESET>::NT))
+    __attribute__((amdgpu_waves_per_eu(MultiCfg<T, PRESET>::WPE, 8)))
+    k_multi_dev(const MultiDescriptor<T>* __restrict__ d, const Bank<T>* __restrict__ bk)
+{
+    using M = MultiCfg<T, PRESET>;
+    __shared__ cx<T> sm[M::LDS];
+    const int tid = threadIdx.x;
+    const int ntasks = d->ntasks;
+    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const int2 task = d->tasks[t];
+        multi_task_phase<T, PRESET>(0, tid, task, d->args, sm, bk);
+        __syncthreads();
+        multi_task_phase<T, PRESET>(1, tid, task, d->args, sm, bk);
+        __syncthreads();
+        multi_task_phase<T, PRESET>(2, tid, task, d->args, sm, bk);
+        __syncthreads();
+    }
+}
+
 // Registry entry of a multi kernel.
 typedef hipError_t (*MultiLaunchFn)(const void* args, const int2* tasks, int ntasks,
                                     const void* bank, hipStream_t stream);
+// Launch of the descriptor-reading form on `grid` workgroups.
+typedef hipError_t (*MultiDevLaunchFn)(const void* descriptor, const void* bank, int grid,
+                                       hipStream_t stream);
 struct MultiDesc {
     int prec;
     int preset;
@@ -143,6 +176,8 @@ struct MultiDesc {
     size_t lds_bytes;
     const char* name;
     MultiLaunchFn launch;
+    MultiDevLaunchFn launch_dev;
+    int (*resident_grid)();  // persistent grid of the descriptor form on the current device
 };
 
 void register_multi(std::vector<MultiDesc>& r);

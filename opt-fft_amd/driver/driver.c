@@ -154,6 +154,7 @@ int main(int argc, char **argv)
         return 3;
     }
     const double s = 1.0 / sqrt((double)V);
+    fft3d_mx_clear_error();
 
     for (int i = 0; i < o.warmup; ++i) run_unit(plan, &o, in, out, pong, count, s);
     gpu_check("warmup");

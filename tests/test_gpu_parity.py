@@ -120,6 +120,7 @@ def test_host_pointer_abi(gpu):
         out = np.zeros_like(x)
         p = L.fft3d_create(n, 2)
         assert p
+        L.fft3d_mx_clear_error()
         L.fft3d_execute(p, x.ctypes.data, out.ctypes.data)
         assert gpu.fft3d.last_error() == ""
         L.fft3d_destroy(p)
