@@ -5,3 +5,4 @@ The package directory name is not a Python identifier; load it by path with
 """
 from . import hip  # noqa: F401
 from . import fft3d  # noqa: F401
+from . import serving  # noqa: F401

@@ -41,7 +41,29 @@ EXPORTED = [
     "fft3d_mx_bank_image", "fft3d_mx_volume_bytes",
     "fft3d_mx_multi_create", "fft3d_mx_multi_execute", "fft3d_mx_multi_get_info",
     "fft3d_mx_multi_destroy",
+    "fft3d_srv_create", "fft3d_srv_submit", "fft3d_srv_wait", "fft3d_srv_run_trace",
+    "fft3d_srv_now", "fft3d_srv_get_stats", "fft3d_srv_destroy",
 ]
+
+
+class SrvConfig(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_size_t), ("streams", ctypes.c_int),
+                ("batched", ctypes.c_int), ("max_items", ctypes.c_int)]
+
+
+class SrvRequest(ctypes.Structure):
+    _fields_ = [("n0", ctypes.c_int), ("n1", ctypes.c_int), ("n2", ctypes.c_int),
+                ("batch", ctypes.c_int), ("direction", ctypes.c_int),
+                ("precision", ctypes.c_int), ("inp", ctypes.c_void_p),
+                ("out", ctypes.c_void_p)]
+
+
+class SrvStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_long), ("completed", ctypes.c_long),
+                ("launches", ctypes.c_long), ("mixed_launches", ctypes.c_long),
+                ("plan_launches", ctypes.c_long),
+                ("mean_items_per_mixed_launch", ctypes.c_double),
+                ("peak_inflight_bytes", ctypes.c_size_t), ("busy_seconds", ctypes.c_double)]
 
 
 class Item(ctypes.Structure):
@@ -106,6 +128,17 @@ def load():
         L.fft3d_mx_multi_execute.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), vp]
         L.fft3d_mx_multi_get_info.argtypes = [vp, ctypes.c_char_p, cs, ip, ip]
         L.fft3d_mx_multi_destroy.argtypes = [vp]
+        L.fft3d_srv_create.argtypes = [ctypes.POINTER(SrvConfig)]
+        L.fft3d_srv_create.restype = vp
+        L.fft3d_srv_submit.argtypes = [vp, ctypes.POINTER(SrvRequest), ctypes.c_double]
+        L.fft3d_srv_submit.restype = cl
+        L.fft3d_srv_wait.argtypes = [vp, cl, ctypes.POINTER(ctypes.c_double)]
+        L.fft3d_srv_run_trace.argtypes = [vp, ctypes.POINTER(SrvRequest),
+                                          ctypes.POINTER(ctypes.c_double), ci,
+                                          ctypes.POINTER(ctypes.c_double)]
+        L.fft3d_srv_now.restype = ctypes.c_double
+        L.fft3d_srv_get_stats.argtypes = [vp, ctypes.POINTER(SrvStats)]
+        L.fft3d_srv_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -331,6 +364,59 @@ class MultiPlan:
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
             load().fft3d_mx_multi_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """fft3d_srv: continuous batching (batched=True) or the plain one-launch-per-request
+    path (batched=False)."""
+
+    def __init__(self, budget_bytes=64 << 30, streams=4, batched=True, max_items=16):
+        cfg = SrvConfig(int(budget_bytes), int(streams), 1 if batched else 0, int(max_items))
+        h = load().fft3d_srv_create(ctypes.byref(cfg))
+        if not h:
+            raise RuntimeError(last_error())
+        self.handle = ctypes.c_void_p(h)
+
+    def submit(self, shape, batch, direction, precision, din, dout=None, arrival=0.0):
+        r = SrvRequest(*[int(n) for n in shape], int(batch), int(direction), int(precision),
+                       din, dout)
+        rid = load().fft3d_srv_submit(self.handle, ctypes.byref(r), float(arrival))
+        if rid < 0:
+            raise ValueError(last_error())
+        return rid
+
+    def wait(self, rid):
+        lat = ctypes.c_double(0)
+        if load().fft3d_srv_wait(self.handle, rid, ctypes.byref(lat)) != 0:
+            raise RuntimeError(last_error())
+        return lat.value
+
+    def run_trace(self, requests, arrivals):
+        """requests: list of SrvRequest; arrivals: seconds from the call.  Latencies (s)."""
+        n = len(requests)
+        reqs = (SrvRequest * n)(*requests)
+        arr = (ctypes.c_double * n)(*[float(a) for a in arrivals])
+        lat = (ctypes.c_double * n)()
+        if load().fft3d_srv_run_trace(self.handle, reqs, arr, n, lat) != 0:
+            raise RuntimeError(last_error())
+        return np.frombuffer(lat, dtype=np.float64).copy()
+
+    def stats(self):
+        st = SrvStats()
+        if load().fft3d_srv_get_stats(self.handle, ctypes.byref(st)) != 0:
+            raise RuntimeError(last_error())
+        return {k: getattr(st, k) for k, _ in SrvStats._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            load().fft3d_srv_destroy(self.handle)
         self.handle = None
 
     def __del__(self):

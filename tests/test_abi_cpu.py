@@ -15,7 +15,7 @@ import oracle
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADERS = [os.path.join(ROOT, "include", h) for h in
-           ("fft3d_api.h", "fft3d_gpu_api.h", "fft3d_mi355x.h")]
+           ("fft3d_api.h", "fft3d_gpu_api.h", "fft3d_mi355x.h", "fft3d_serve.h")]
 
 
 def declared_functions():
