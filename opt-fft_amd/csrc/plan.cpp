@@ -76,7 +76,7 @@ static void wisdom_store_locked(const std::string& key, const std::string& name)
     fclose(f);
 }
 
-static std::string wisdom_key(int n0, int n1, int n2, int prec, int dir, long batch)
+static std::string wisdom_key(int n0, int n1, int n2, int prec, int dir, long batch, int algo)
 {
     int dev = 0;
     hipDeviceProp_t prop;
@@ -91,8 +91,9 @@ static std::string wisdom_key(int n0, int n1, int n2, int prec, int dir, long ba
     int bclass = 0;
     while ((1L << (bclass + 1)) <= batch) ++bclass;
     char buf[256];
-    snprintf(buf, sizeof buf, "%s/%dcu/%dx%dx%d/%s/%s/b2^%d", arch.c_str(), cus, n0, n1, n2,
-             prec == kC128 ? "c128" : "c64", dir < 0 ? "fwd" : "bwd", bclass);
+    snprintf(buf, sizeof buf, "%s/%dcu/%dx%dx%d/%s/%s/b2^%d%s", arch.c_str(), cus, n0, n1, n2,
+             prec == kC128 ? "c128" : "c64", dir < 0 ? "fwd" : "bwd", bclass,
+             algo == 1 ? "/mfma-bf16" : algo == 2 ? "/mfma-fp16" : "");
     return buf;
 }
 
@@ -180,11 +181,13 @@ struct fft3d_mx_plan {
 extern "C" const char* fft3d_mx_last_error(void) { return last_error(); }
 extern "C" void fft3d_mx_clear_error(void) { clear_error(); }
 
+// algo < 0 accepts every algorithm (lookups by name).
 static void candidates(int n0, int n1, int n2, int prec, int dir,
-                       std::vector<const KernelDesc*>& out)
+                       std::vector<const KernelDesc*>& out, int algo = 0)
 {
     for (const KernelDesc& k : registry())
-        if (k.n0 == n0 && k.n1 == n1 && k.n2 == n2 && k.prec == prec && k.dir == dir)
+        if (k.n0 == n0 && k.n1 == n1 && k.n2 == n2 && k.prec == prec && k.dir == dir &&
+            (algo < 0 || k.algo == algo))
             out.push_back(&k);
 }
 
@@ -217,17 +220,24 @@ static fft3d_mx_plan* create_impl(int n0, int n1, int n2, int batch, int directi
         fail(FFT3D_MX_EINVAL, "fft3d_mx_create: unknown precision %d", precision);
         return nullptr;
     }
-    std::vector<const KernelDesc*> cands;
-    candidates(n0, n1, n2, precision, direction, cands);
+    if ((flags & FFT3D_MX_MFMA_BF16) && (flags & FFT3D_MX_MFMA_FP16)) {
+        fail(FFT3D_MX_EINVAL, "fft3d_mx_create: MFMA_BF16 and MFMA_FP16 are exclusive");
+        return nullptr;
+    }
+    const int algo = (flags & FFT3D_MX_MFMA_BF16) ? 1 : (flags & FFT3D_MX_MFMA_FP16) ? 2 : 0;
+    std::vector<const KernelDesc*> cands, any;
+    candidates(n0, n1, n2, precision, direction, cands, algo);
+    candidates(n0, n1, n2, precision, direction, any, -1);
     if (cands.empty()) {
-        fail(FFT3D_MX_EUNSUPPORTED, "fft3d_mx_create: no kernel for %dx%dx%d %s", n0, n1, n2,
+        fail(FFT3D_MX_EUNSUPPORTED, "fft3d_mx_create: no %skernel for %dx%dx%d %s",
+             algo ? "matrix-core " : "", n0, n1, n2,
              precision == kC128 ? "complex128" : "complex64");
         return nullptr;
     }
     if (!forced || !*forced) forced = getenv("FFT3D_MX_KERNEL");
     const KernelDesc* pick = nullptr;
     if (forced && *forced) {
-        for (const KernelDesc* k : cands)
+        for (const KernelDesc* k : any)
             if (!strcmp(k->name, forced)) pick = k;
         if (!pick && getenv("FFT3D_MX_KERNEL") != forced) {
             fail(FFT3D_MX_EUNSUPPORTED, "fft3d_mx_create: kernel '%s' does not serve %dx%dx%d",
@@ -272,7 +282,7 @@ static fft3d_mx_plan* create_impl(int n0, int n1, int n2, int batch, int directi
         if (cands.size() == 1 || (flags & FFT3D_MX_NO_TUNE) || batch == 0) {
             pick = cands[0];
         } else {
-            const std::string key = wisdom_key(n0, n1, n2, precision, direction, batch);
+            const std::string key = wisdom_key(n0, n1, n2, precision, direction, batch, algo);
             std::lock_guard<std::mutex> g(g_wisdom_mu);
             wisdom_load_locked();
             auto it = g_wisdom.find(key);

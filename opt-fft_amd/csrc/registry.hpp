@@ -29,12 +29,14 @@ struct KernelDesc {
     size_t lds_bytes;
     const char* name;
     LaunchFn launch;
+    int algo = 0;    // 0 exact FP FFT; 1 DFT matrix on MFMA, split bf16; 2 split fp16
 };
 
 // Appended by each kernel translation unit.
 void register_fused_small(std::vector<KernelDesc>& r);   // 6, 8, 12, 8x8x16
 void register_fused_17(std::vector<KernelDesc>& r);      // 17
 void register_pencil(std::vector<KernelDesc>& r);        // 24, 36, 24x24x48
+void register_mfma(std::vector<KernelDesc>& r);          // complex64 DFT-as-matrix
 
 const std::vector<KernelDesc>& registry();
 

@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libfft3d_mi355x.so")
 
 FORWARD, BACKWARD = -1, +1
 C128, C64 = 0, 1
-NORMALIZE, NO_TUNE = 1, 2
+NORMALIZE, NO_TUNE, MFMA_BF16, MFMA_FP16 = 1, 2, 4, 8
 
 # Every symbol the three public headers declare (include/*.h).
 EXPORTED = [

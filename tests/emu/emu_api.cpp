@@ -32,7 +32,7 @@ extern "C" int emu_transform(const char* name, const void* in, void* out, long n
         bank_host_image(k.prec, img);
         g_emu_reverse = reverse;
         g_emu_bad_block = -1;
-        (void)k.launch(in, out, nvol, img.data(), scale, nullptr);
+        if (k.launch(in, out, nvol, img.data(), scale, nullptr) != hipSuccess) return -2;
         return g_emu_bad_block >= 0 ? (int)(-1000 - g_emu_bad_block) : 0;
     }
     return -1;

@@ -47,6 +47,8 @@ def test_every_variant_on_the_emulator(emu):
     assert n >= 100
     for i in range(n):
         name = emu.emu_kernel_name(i).decode()
+        if name.startswith("mfma_"):
+            continue  # wave-level matrix-core kernels: checked on the GPU only
         shape, prec, d, per_wg = parse(name)
         dt = np.complex128 if prec == 0 else np.complex64
         tol = 1e-13 if prec == 0 else 2e-6

@@ -1,0 +1,59 @@
+"""Matrix-core DFT-as-matrix variant (complex64 only, opt-in by plan flag).
+
+Each axis pass is a real (2N x 2N) x (2N x 16) product per 16 lines on
+v_mfma_f32_16x16x16 instructions, both operands split into high + low half-precision parts,
+hh + hl + lh accumulated in FP32.  Checked against the exact oracle; the bars below are the
+variants' stated accuracy, looser than the FFT path's 2e-6."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(6, 6, 6), (8, 8, 8), (12, 12, 12), (17, 17, 17), (8, 8, 16)]
+BARS = {"bf16": 1e-4, "fp16": 1e-5}
+
+
+@pytest.mark.parametrize("split", ["bf16", "fp16"])
+def test_mfma_plans_match_oracle(gpu, split):
+    f = gpu.fft3d
+    flag = f.MFMA_BF16 if split == "bf16" else f.MFMA_FP16
+    for shape in SHAPES:
+        for direction in (f.FORWARD, f.BACKWARD):
+            for batch in (1, 33):
+                x = oracle.make_input(shape, batch, seed=7 + batch, dist="uniform",
+                                      dtype=np.complex64)
+                plan = f.Plan(shape, batch, direction, f.C64, flags=flag)
+                assert plan.kernel.startswith("mfma_") and plan.kernel.endswith(split + "x3")
+                y = plan.run(x)
+                plan.close()
+                err = oracle.rel_l2(y, oracle.fftn_numpy(x.astype(np.complex128), sign=direction))
+                print(f"{plan.kernel} dir={direction} B={batch} rel_l2={err:.2e}", flush=True)
+                assert np.all(np.isfinite(y))
+                assert err <= BARS[split], (shape, direction, batch, err)
+
+
+def test_mfma_normalized_round_trip(gpu):
+    f = gpu.fft3d
+    x = oracle.make_input((17, 17, 17), 8, seed=3, dtype=np.complex64)
+    fwd = f.Plan((17, 17, 17), 8, f.FORWARD, f.C64, flags=f.MFMA_FP16)
+    bwd = f.Plan((17, 17, 17), 8, f.BACKWARD, f.C64, flags=f.MFMA_FP16 | f.NORMALIZE)
+    back = bwd.run(fwd.run(x))
+    assert oracle.rel_l2(back, x) <= 2 * BARS["fp16"]
+    fwd.close()
+    bwd.close()
+
+
+def test_mfma_flags_rejected_where_unsupported(gpu):
+    f = gpu.fft3d
+    with pytest.raises(ValueError):
+        f.Plan((17, 17, 17), 1, f.FORWARD, f.C128, flags=f.MFMA_BF16)
+    with pytest.raises(ValueError):
+        f.Plan((36, 36, 36), 1, f.FORWARD, f.C64, flags=f.MFMA_BF16)
+    with pytest.raises(ValueError):
+        f.Plan((8, 8, 8), 1, f.FORWARD, f.C64, flags=f.MFMA_BF16 | f.MFMA_FP16)
+    # without the flag a complex64 plan never picks a matrix-core kernel
+    p = f.Plan((8, 8, 8), 64, f.FORWARD, f.C64)
+    assert not p.kernel.startswith("mfma_")
+    p.close()

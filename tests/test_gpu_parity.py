@@ -12,6 +12,17 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 TOL = {0: 1e-13, 1: 2e-6}
+# Matrix-core (DFT-as-matrix) complex64 variants: operands split into two half-precision
+# parts, three products accumulated in FP32.  Per pass the represented operands carry ~16
+# (bf16 x2) or ~22 (fp16 x2) significant bits; three passes of an N-term sum.
+TOL_MFMA = {"bf16x3": 1e-4, "fp16x3": 1e-5}
+
+
+def _tol(name, prec):
+    for suffix, t in TOL_MFMA.items():
+        if name.endswith(suffix):
+            return t
+    return TOL[prec]
 
 
 def _kernels(pkg):
@@ -36,7 +47,7 @@ def test_every_kernel_variant_matches_oracle(gpu):
             err = oracle.rel_l2(y, ref)
             print(f"{name} B={batch} rel_l2={err:.2e}", flush=True)
             assert np.all(np.isfinite(y)), name
-            assert err <= TOL[prec], f"{name} B={batch}: rel_l2 {err:.3e}"
+            assert err <= _tol(name, prec), f"{name} B={batch}: rel_l2 {err:.3e}"
             worst[name] = max(worst.get(name, 0.0), err)
 
 
