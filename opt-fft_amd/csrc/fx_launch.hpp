@@ -43,9 +43,17 @@ long persistent_grid(F f, int nt, int cap, long nblocks)
     return g < nblocks ? g : nblocks;
 }
 
+// Dynamic block claiming pays when workgroups walk several blocks each; when the grid
+// covers (nearly) every block, the one-per-workgroup failing claims would only queue on a
+// single address, so those launches walk a fixed stride instead.
+inline unsigned* claim_counter(unsigned* work, long grid, long nblocks)
+{
+    return nblocks >= 2 * grid ? work : nullptr;
+}
+
 template <class C>
 hipError_t launch_fused(const void* in, void* out, long nvol, const void* bank, double scl,
-                        hipStream_t stream)
+                        unsigned*, hipStream_t stream)
 {
     using T = typename C::T;
     if (nvol <= 0) return hipSuccess;
@@ -57,21 +65,23 @@ hipError_t launch_fused(const void* in, void* out, long nvol, const void* bank, 
 
 template <class C, int PERSIST>
 hipError_t launch_fpipe(const void* in, void* out, long nvol, const void* bank, double scl,
-                        hipStream_t stream)
+                        unsigned* work, hipStream_t stream)
 {
     using T = typename C::T;
     if (nvol <= 0) return hipSuccess;
     const long nblocks = (nvol + C::VPW - 1) / C::VPW;
-    const long grid = PERSIST > 0 ? persistent_grid(k_fpipe<C>, C::NT, PERSIST, nblocks) : nblocks;
-    hipLaunchKernelGGL((k_fpipe<C>), dim3((unsigned)grid), dim3(C::NT), 0, stream,
+    if (nblocks >= (1L << 31)) return hipErrorInvalidValue;
+    using P = PipeCfg<C, PERSIST>;
+    const long grid = PERSIST > 0 ? persistent_grid(k_fpipe<P>, C::NT, PERSIST, nblocks) : nblocks;
+    hipLaunchKernelGGL((k_fpipe<P>), dim3((unsigned)grid), dim3(C::NT), 0, stream,
                        (const cx<T>*)in, (cx<T>*)out, nvol, nblocks, (const Bank<T>*)bank,
-                       (T)scl);
+                       (T)scl, claim_counter(work, grid, nblocks));
     return hipGetLastError();
 }
 
 template <class C>
 hipError_t launch_pencil(const void* in, void* out, long nvol, const void* bank, double scl,
-                         hipStream_t stream)
+                         unsigned* work, hipStream_t stream)
 {
     using T = typename C::T;
     if (nvol <= 0) return hipSuccess;
@@ -84,8 +94,10 @@ hipError_t launch_pencil(const void* in, void* out, long nvol, const void* bank,
         const long nblocks = (nslab + C::SPW - 1) / C::SPW;
         if constexpr (C::PIPE > 0) {
             const long ga = persistent_grid(k_spipe<C>, C::NT, C::PIPE, nblocks);
+            if (nblocks >= (1L << 31)) return hipErrorInvalidValue;
             hipLaunchKernelGGL((k_spipe<C>), dim3((unsigned)ga), dim3(C::NT), 0, stream, src,
-                               dst, nslab, nblocks, (const Bank<T>*)bank);
+                               dst, nslab, nblocks, (const Bank<T>*)bank,
+                               claim_counter(work, ga, nblocks));
         } else {
             hipLaunchKernelGGL((k_slab<C>), dim3((unsigned)nblocks), dim3(C::NT), 0, stream,
                                src, dst, nslab, (const Bank<T>*)bank);

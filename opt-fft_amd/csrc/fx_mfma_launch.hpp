@@ -5,7 +5,7 @@ namespace fx {
 
 template <class C, int SPLIT>
 hipError_t launch_mfma(const void* in, void* out, long nvol, const void* bank, double scl,
-                       hipStream_t stream)
+                       unsigned*, hipStream_t stream)
 {
     if (nvol <= 0) return hipSuccess;
     const long grid = (nvol + C::VPW - 1) / C::VPW;

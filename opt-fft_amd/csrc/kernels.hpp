@@ -290,24 +290,49 @@ struct FusedPipe {
     }
 };
 
+// A pipelined kernel launched with at most P workgroups per CU (P > 0) may spend the
+// registers of P resident workgroups rather than of as many as the LDS footprint allows.
+template <class C, int P>
+struct PipeCfg : C {
+    static constexpr int WPE =
+        P > 0 ? waves_per_eu(C::LDS * (int)sizeof(cx<typename C::T>), C::NT, P) : C::WPE;
+};
+
+// Next block of a persistent workgroup.  Each workgroup starts at block blockIdx.x and then
+// claims one ticket per iteration until it draws a block >= nblocks, so a launch with
+// grid <= nblocks draws exactly nblocks tickets, 0 .. nblocks-1: the holder of the last one
+// puts the counter back to zero for the next launch.  Claiming hands blocks to whichever
+// workgroup is free first; a fixed stride (work == NULL) lets the slowest CU set the time.
+__device__ __forceinline__ long claim_block(unsigned* work, long nblocks, long b)
+{
+    if (!work) return b + gridDim.x;
+    const unsigned t = atomicAdd(work, 1u);
+    if ((long)t == nblocks - 1) (void)atomicExch(work, 0u);
+    return (long)gridDim.x + (long)t;
+}
+
 template <class C>
 __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE, 8)))
     k_fpipe(const cx<typename C::T>* __restrict__ in, cx<typename C::T>* __restrict__ out,
             long nvol, long nblocks, const Bank<typename C::T>* __restrict__ bk,
-            typename C::T scl)
+            typename C::T scl, unsigned* __restrict__ work)
 {
     using K = FusedPipe<C>;
     __shared__ cx<typename C::T> sm[C::LDS];
+    __shared__ long s_next;
     typename K::Regs R;
     const int tid = threadIdx.x;
     long b = blockIdx.x;
     if (b >= nblocks) return;
     K::load_x(tid, b, nvol, in, R);
     for (;;) {
+        long claim = 0;
+        if (tid == 0) claim = claim_block(work, nblocks, b);
         K::xpass(tid, b, nvol, R, sm, bk);
-        const long nb = b + gridDim.x;
-        if (nb < nblocks) K::load_x(tid, nb, nvol, in, R);
+        if (tid == 0) s_next = claim;
         __syncthreads();
+        const long nb = s_next;
+        if (nb < nblocks) K::load_x(tid, nb, nvol, in, R);
         K::zpass(tid, b, nvol, sm, bk);
         __syncthreads();
         K::ypass(tid, b, nvol, sm, out, bk, scl);
@@ -489,20 +514,25 @@ struct SlabPipe {
 template <class C>
 __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE, 8)))
     k_spipe(const cx<typename C::T>* __restrict__ in, cx<typename C::T>* __restrict__ out,
-            long nslab, long nblocks, const Bank<typename C::T>* __restrict__ bk)
+            long nslab, long nblocks, const Bank<typename C::T>* __restrict__ bk,
+            unsigned* __restrict__ work)
 {
     using K = SlabPipe<C>;
     __shared__ cx<typename C::T> sm[C::LDS];
+    __shared__ long s_next;
     typename K::Regs R;
     const int tid = threadIdx.x;
     long b = blockIdx.x;
     if (b >= nblocks) return;
     K::load_y(tid, b, nslab, in, R);
     for (;;) {
+        long claim = 0;
+        if (tid == 0) claim = claim_block(work, nblocks, b);
         K::ypass(tid, b, nslab, R, sm, bk);
-        const long nb = b + gridDim.x;
-        if (nb < nblocks) K::load_y(tid, nb, nslab, in, R);
+        if (tid == 0) s_next = claim;
         __syncthreads();
+        const long nb = s_next;
+        if (nb < nblocks) K::load_y(tid, nb, nslab, in, R);
         K::zpass(tid, b, nslab, sm, bk);
         __syncthreads();
         K::store(tid, b, nslab, sm, out);

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -100,15 +101,16 @@ static std::string wisdom_key(int n0, int n1, int n2, int prec, int dir, long ba
 // Time one candidate: 2 warm launches, then the best of 3 groups of 5.  Seconds per
 // launch, or -1 if it failed to launch.
 static double trial(const KernelDesc& k, const void* din, void* dout, long nvol,
-                    const void* bank, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+                    const void* bank, unsigned* work, hipStream_t s, hipEvent_t e0,
+                    hipEvent_t e1)
 {
     for (int i = 0; i < 2; ++i)
-        if (k.launch(din, dout, nvol, bank, 1.0, s) != hipSuccess) return -1.0;
+        if (k.launch(din, dout, nvol, bank, 1.0, work, s) != hipSuccess) return -1.0;
     double best = 1e30;
     for (int g = 0; g < 3; ++g) {
         if (hipEventRecord(e0, s) != hipSuccess) return -1.0;
         for (int i = 0; i < 5; ++i)
-            if (k.launch(din, dout, nvol, bank, 1.0, s) != hipSuccess) return -1.0;
+            if (k.launch(din, dout, nvol, bank, 1.0, work, s) != hipSuccess) return -1.0;
         if (hipEventRecord(e1, s) != hipSuccess) return -1.0;
         if (hipEventSynchronize(e1) != hipSuccess) return -1.0;
         float ms = 0.f;
@@ -120,7 +122,8 @@ static double trial(const KernelDesc& k, const void* din, void* dout, long nvol,
 }
 
 static const KernelDesc* race(const std::vector<const KernelDesc*>& cands, int n0, int n1,
-                              int n2, int prec, long batch, const void* bank, double* best_s)
+                              int n2, int prec, long batch, const void* bank, unsigned* work,
+                              double* best_s)
 {
     const size_t vol_bytes = (size_t)n0 * n1 * n2 * elem_bytes(prec);
     // Cap the race buffers at ~256 MiB each so planning stays bounded; at least one
@@ -144,7 +147,7 @@ static const KernelDesc* race(const std::vector<const KernelDesc*>& cands, int n
         if (hipEventCreate(&e0) == hipSuccess) {
             if (hipEventCreate(&e1) == hipSuccess) {
                 for (const KernelDesc* k : cands) {
-                    const double t = trial(*k, din, dout, rb, bank, s, e0, e1);
+                    const double t = trial(*k, din, dout, rb, bank, work, s, e0, e1);
                     if (t > 0 && t < wt) {
                         wt = t;
                         win = k;
@@ -168,15 +171,29 @@ static const KernelDesc* race(const std::vector<const KernelDesc*>& cands, int n
 
 using namespace fx;
 
+// Claim counters of the persistent kernels: one 64 B line per slot, slots taken round-robin
+// by successive launches, so up to kWorkSlots launches of one plan may be in flight at once
+// (on any streams) without sharing a counter.
+constexpr int kWorkSlots = 64;
+constexpr int kWorkStride = 16;  // unsigned per slot
+
 struct fft3d_mx_plan {
     int n0, n1, n2, batch, dir, prec;
     unsigned flags;
     const KernelDesc* k;
     void* bank;
+    unsigned* work;
+    std::atomic<unsigned> work_next;
     double scale;
     double tune_us;
     int device;
 };
+
+static unsigned* next_work(fft3d_mx_plan* p)
+{
+    const unsigned i = p->work_next.fetch_add(1u, std::memory_order_relaxed) % kWorkSlots;
+    return p->work + (size_t)i * kWorkStride;
+}
 
 extern "C" const char* fft3d_mx_last_error(void) { return last_error(); }
 extern "C" void fft3d_mx_clear_error(void) { clear_error(); }
@@ -277,6 +294,16 @@ static fft3d_mx_plan* create_impl(int n0, int n1, int n2, int batch, int directi
         delete p;
         return nullptr;
     }
+    const size_t work_bytes = sizeof(unsigned) * kWorkSlots * kWorkStride;
+    e = hipMalloc((void**)&p->work, work_bytes);
+    if (e == hipSuccess) e = hipMemset(p->work, 0, work_bytes);
+    if (e != hipSuccess) {
+        fail(FFT3D_MX_ENOMEM, "fft3d_mx_create: claim counters: %s", hipGetErrorString(e));
+        (void)hipFree(p->bank);
+        if (p->work) (void)hipFree(p->work);
+        delete p;
+        return nullptr;
+    }
 
     if (!pick) {
         if (cands.size() == 1 || (flags & FFT3D_MX_NO_TUNE) || batch == 0) {
@@ -291,7 +318,7 @@ static fft3d_mx_plan* create_impl(int n0, int n1, int n2, int batch, int directi
                     if (it->second == k->name) pick = k;
             if (!pick) {
                 double best_s = 0.0;
-                pick = race(cands, n0, n1, n2, precision, batch, p->bank, &best_s);
+                pick = race(cands, n0, n1, n2, precision, batch, p->bank, p->work, &best_s);
                 p->tune_us = best_s * 1e6;
                 wisdom_store_locked(key, pick->name);
             }
@@ -332,7 +359,8 @@ extern "C" int fft3d_mx_execute_n(fft3d_mx_plan* p, const void* in, void* out, l
     int cur = p->device;
     (void)hipGetDevice(&cur);
     if (cur != p->device) (void)hipSetDevice(p->device);
-    const hipError_t e = p->k->launch(in, out, batch, p->bank, p->scale, (hipStream_t)stream);
+    const hipError_t e =
+        p->k->launch(in, out, batch, p->bank, p->scale, next_work(p), (hipStream_t)stream);
     if (cur != p->device) (void)hipSetDevice(cur);
     if (e != hipSuccess)
         return fail(FFT3D_MX_EHIP, "fft3d_mx_execute: %s: %s", p->k->name, hipGetErrorString(e));
@@ -362,7 +390,8 @@ extern "C" int fft3d_mx_execute_chain(fft3d_mx_plan* p, const void* in, void* ou
     void* dst = (m % 2) ? out : pong;
     hipError_t e = hipSuccess;
     for (int s = 0; s < m && e == hipSuccess; ++s) {
-        e = p->k->launch(src, dst, p->batch, p->bank, step_scale, (hipStream_t)stream);
+        e = p->k->launch(src, dst, p->batch, p->bank, step_scale, next_work(p),
+                         (hipStream_t)stream);
         src = dst;
         dst = dst == out ? pong : out;
     }
@@ -377,6 +406,7 @@ extern "C" void fft3d_mx_destroy(fft3d_mx_plan* p)
 {
     if (!p) return;
     (void)hipFree(p->bank);
+    (void)hipFree(p->work);
     delete p;
 }
 

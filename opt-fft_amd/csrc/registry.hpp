@@ -13,9 +13,12 @@ enum Kind { kFused = 0, kPencil = 1 };
 
 // Launch on `stream`: transform nvol contiguous volumes from `in` to `out` (device
 // pointers, out-of-place), multiplying every output by `scale`.  `bank` is the device
-// Bank<T> of matching precision.  Returns the launch status.
+// Bank<T> of matching precision.  `work` is a device claim counter that is zero when the
+// launch starts and that the launch leaves at zero (persistent kernels hand out blocks
+// from it); launches that share one counter must be ordered on one stream.  NULL makes
+// persistent kernels walk their blocks in a fixed order instead.  Returns the launch status.
 typedef hipError_t (*LaunchFn)(const void* in, void* out, long nvol, const void* bank,
-                               double scale, hipStream_t stream);
+                               double scale, unsigned* work, hipStream_t stream);
 
 struct KernelDesc {
     int n0, n1, n2;

@@ -156,7 +156,7 @@ static size_t vol_bytes(const fft3d_srv_request& r)
 }
 
 // Launch a mixed batch of fused requests on a free slot of their precision.
-static bool launch_mixed(fft3d_srv* s, std::vector<Req*>& reqs, int stream)
+static int launch_mixed(fft3d_srv* s, std::vector<Req*>& reqs, int stream)
 {
     const int prec = reqs[0]->r.precision;
     Slot* slot = nullptr;
@@ -167,7 +167,7 @@ static bool launch_mixed(fft3d_srv* s, std::vector<Req*>& reqs, int stream)
             si = (int)i;
             break;
         }
-    if (!slot) return false;
+    if (!slot) return 0;
     const MultiDesc& d = *s->mk[prec];
     int code[kMultiMaxItems], batch[kMultiMaxItems];
     const int n = (int)reqs.size();
@@ -197,8 +197,8 @@ static bool launch_mixed(fft3d_srv* s, std::vector<Req*>& reqs, int stream)
                                                : fill((MultiDescriptor<float>*)slot->host);
     hipStream_t st = s->streams[stream];
     if (hipMemcpyAsync(slot->dev, slot->host, bytes, hipMemcpyHostToDevice, st) != hipSuccess)
-        return false;
-    if (hipGraphLaunch(slot->exec, st) != hipSuccess) return false;
+        return -1;
+    if (hipGraphLaunch(slot->exec, st) != hipSuccess) return -1;
     Launch L{s->event_pool.back(), stream, si, reqs};
     s->event_pool.pop_back();
     (void)hipEventRecord(L.ev, st);
@@ -208,7 +208,7 @@ static bool launch_mixed(fft3d_srv* s, std::vector<Req*>& reqs, int stream)
     s->st.launches++;
     s->st.mixed_launches++;
     s->items_in_mixed += n;
-    return true;
+    return 1;
 }
 
 static bool launch_plan(fft3d_srv* s, Req* q, int stream, bool sync)
@@ -229,19 +229,40 @@ static bool launch_plan(fft3d_srv* s, Req* q, int stream, bool sync)
     return true;
 }
 
-// t < 0 marks a request that could not be launched.
+// Copy the scheduler's counters to the snapshot fft3d_srv_get_stats reads.
+static void publish(fft3d_srv* s)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->st_pub = s->st;
+    s->st_pub.submitted = s->submitted;
+    s->st_pub.mean_items_per_mixed_launch =
+        s->st.mixed_launches ? (double)s->items_in_mixed / (double)s->st.mixed_launches : 0.0;
+}
+
+// t < 0 marks a request that could not be launched.  The waiter is released by the caller
+// (release()) once the statistics that count this completion are published.
 static void finish(fft3d_srv* s, Req* q, double t)
 {
     q->t_done = t;
     s->inflight_bytes -= q->io_bytes;
     if (q->arena_out) s->arena.release(q->out, q->out_bytes);
     s->st.completed++;
-    q->done.store(1, std::memory_order_release);
+}
+
+static void release(Req* q) { q->done.store(1, std::memory_order_release); }
+
+// Requests that could not be launched complete at once with a failure mark.
+static void fail_reqs(fft3d_srv* s, const std::vector<Req*>& reqs)
+{
+    for (Req* q : reqs) finish(s, q, -1.0);
+    publish(s);
+    for (Req* q : reqs) release(q);
+    s->cv_done.notify_all();
 }
 
 static void retire(fft3d_srv* s)
 {
-    bool any = false;
+    std::vector<Req*> done;
     for (size_t i = 0; i < s->inflight.size();) {
         Launch& L = s->inflight[i];
         if (hipEventQuery(L.ev) != hipSuccess) {
@@ -249,15 +270,20 @@ static void retire(fft3d_srv* s)
             continue;
         }
         const double t = steady_now();
-        for (Req* q : L.reqs) finish(s, q, t);
+        for (Req* q : L.reqs) {
+            finish(s, q, t);
+            done.push_back(q);
+        }
         if (L.slot >= 0) s->slots[L.slot].busy = false;
         s->stream_busy[L.stream] = false;
         s->event_pool.push_back(L.ev);
         s->inflight[i] = s->inflight.back();
         s->inflight.pop_back();
-        any = true;
     }
-    if (any) s->cv_done.notify_all();
+    if (done.empty()) return;
+    publish(s);
+    for (Req* q : done) release(q);
+    s->cv_done.notify_all();
 }
 
 static int free_stream(fft3d_srv* s)
@@ -293,7 +319,7 @@ static void dispatch(fft3d_srv* s)
         Req* head = s->ready.front();
         if (!s->cfg.batched || head->code < 0) {
             s->ready.pop_front();
-            if (!launch_plan(s, head, stream, !s->cfg.batched)) finish(s, head, -1.0);
+            if (!launch_plan(s, head, stream, !s->cfg.batched)) fail_reqs(s, {head});
             if (!s->cfg.batched) retire(s);
             continue;
         }
@@ -307,7 +333,10 @@ static void dispatch(fft3d_srv* s)
                 ++it;
             }
         }
-        if (!launch_mixed(s, pack, stream)) {
+        const int rc = launch_mixed(s, pack, stream);
+        if (rc < 0) {
+            fail_reqs(s, pack);
+        } else if (rc == 0) {
             // no free slot: put them back in order and wait for a retirement
             for (auto it = pack.rbegin(); it != pack.rend(); ++it) s->ready.push_front(*it);
             return;
@@ -340,14 +369,7 @@ static void scheduler(fft3d_srv* s)
             s->st.busy_seconds += t - s->busy_since;
             s->busy_since = -1;
         }
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->st_pub = s->st;
-            s->st_pub.submitted = s->submitted;
-            s->st_pub.mean_items_per_mixed_launch =
-                s->st.mixed_launches ? (double)s->items_in_mixed / (double)s->st.mixed_launches
-                                     : 0.0;
-        }
+        publish(s);
         if (!s->inflight.empty()) std::this_thread::yield();
     }
 }

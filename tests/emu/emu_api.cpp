@@ -21,8 +21,9 @@ extern "C" const char* emu_kernel_name(int i)
     return (i >= 0 && i < (int)registry().size()) ? registry()[i].name : nullptr;
 }
 
-// Run kernel `name` on host buffers.  Returns 0, -1 for an unknown name, or
-// -1000 - block when a block wrote outside its LDS allocation.
+// Run kernel `name` on host buffers.  Returns 0, -1 for an unknown name, -2 when the
+// launcher refuses, -3 when the claim counter is not back at zero, or -1000 - block when a
+// block wrote outside its LDS allocation.
 extern "C" int emu_transform(const char* name, const void* in, void* out, long nvol,
                              double scale, int reverse)
 {
@@ -32,8 +33,10 @@ extern "C" int emu_transform(const char* name, const void* in, void* out, long n
         bank_host_image(k.prec, img);
         g_emu_reverse = reverse;
         g_emu_bad_block = -1;
-        if (k.launch(in, out, nvol, img.data(), scale, nullptr) != hipSuccess) return -2;
-        return g_emu_bad_block >= 0 ? (int)(-1000 - g_emu_bad_block) : 0;
+        unsigned work = 0;  // the launch must hand it back at zero
+        if (k.launch(in, out, nvol, img.data(), scale, &work, nullptr) != hipSuccess) return -2;
+        if (g_emu_bad_block >= 0) return (int)(-1000 - g_emu_bad_block);
+        return work == 0 ? 0 : -3;
     }
     return -1;
 }

@@ -20,7 +20,7 @@ int main()
         if (k.algo != 0) continue;  // matrix-core kernels are not emulated
         const size_t V = (size_t)k.n0 * k.n1 * k.n2;
         const size_t eb = k.prec == kC128 ? 16 : 8;
-        const long batches[] = {1, 2, (long)k.vpw + 1, 2L * k.vpw + 3};
+        const long batches[] = {1, 2, (long)k.vpw + 1, 2L * k.vpw + 3, 7L * k.vpw + 1};
         for (long B : batches) {
             const size_t bytes = V * (size_t)B * eb;
             unsigned char* in = (unsigned char*)malloc(bytes);

@@ -52,9 +52,48 @@ inline void emu_threads(int nt, F f)
         for (int t = 0; t < nt; ++t) f(t);
 }
 
+// Host mirror of the device claim_block (kernels.hpp): the persistent workgroup at block b
+// draws its next block from the launch's counter; the holder of ticket nblocks-1 resets it.
+inline long emu_claim(unsigned* work, long nblocks, long grid, long b)
+{
+    if (!work) return b + grid;
+    const unsigned t = (*work)++;
+    if ((long)t == nblocks - 1) *work = 0u;
+    return grid + (long)t;
+}
+
+// Emulated persistent launch: `grid` workgroups, each starting at its own index, advanced
+// one iteration at a time in round-robin order (or reversed), so claims interleave as they
+// do on the device.  `start(w, b)` loads the first block of workgroup w, `iter(w, b, nb)`
+// runs one iteration on block b while prefetching nb (if nb < nblocks).
+template <typename Start, typename Iter>
+inline void emu_persistent(long nblocks, long grid, unsigned* work, Start start, Iter iter)
+{
+    std::vector<long> cur(grid);
+    std::vector<char> live(grid, 1);
+    for (long w = 0; w < grid; ++w) {
+        cur[w] = w;
+        start(w, w);
+    }
+    long left = grid;
+    while (left > 0)
+        for (long i = 0; i < grid; ++i) {
+            const long w = g_emu_reverse ? grid - 1 - i : i;
+            if (!live[w]) continue;
+            const long nb = emu_claim(work, nblocks, grid, cur[w]);
+            iter(w, cur[w], nb);
+            if (nb >= nblocks) {
+                live[w] = 0;
+                --left;
+            } else {
+                cur[w] = nb;
+            }
+        }
+}
+
 template <class C>
 hipError_t launch_fused(const void* in, void* out, long nvol, const void* bank, double scl,
-                        hipStream_t)
+                        unsigned*, hipStream_t)
 {
     using T = typename C::T;
     using K = Fused<C>;
@@ -81,7 +120,7 @@ constexpr long kEmuPersistentGrid = 3;
 
 template <class C, int PERSIST>
 hipError_t launch_fpipe(const void* in, void* out, long nvol, const void* bank, double scl,
-                        hipStream_t)
+                        unsigned* work, hipStream_t)
 {
     using T = typename C::T;
     using K = FusedPipe<C>;
@@ -94,29 +133,26 @@ hipError_t launch_fpipe(const void* in, void* out, long nvol, const void* bank, 
     const cx<T>* src = (const cx<T>*)in;
     cx<T>* dst = (cx<T>*)out;
     EmuLds<T> lds(C::LDS);
-    std::vector<Box> R(C::NT);
-    for (long w = 0; w < grid; ++w) {
-        long b = w;
-        emu_threads(C::NT, [&](int t) { K::load_x(t, b, nvol, src, R[t].r); });
-        for (;;) {
+    std::vector<std::vector<Box>> R(grid, std::vector<Box>(C::NT));
+    emu_persistent(
+        nblocks, grid, PERSIST > 0 && nblocks >= 2 * grid ? work : nullptr,
+        [&](long w, long b) { emu_threads(C::NT, [&](int t) { K::load_x(t, b, nvol, src, R[w][t].r); }); },
+        [&](long w, long b, long nb) {
             lds.poison();
             cx<T>* sm = lds.data();
-            emu_threads(C::NT, [&](int t) { K::xpass(t, b, nvol, R[t].r, sm, bk); });
-            const long nb = b + grid;
-            if (nb < nblocks) emu_threads(C::NT, [&](int t) { K::load_x(t, nb, nvol, src, R[t].r); });
+            emu_threads(C::NT, [&](int t) { K::xpass(t, b, nvol, R[w][t].r, sm, bk); });
+            if (nb < nblocks)
+                emu_threads(C::NT, [&](int t) { K::load_x(t, nb, nvol, src, R[w][t].r); });
             emu_threads(C::NT, [&](int t) { K::zpass(t, b, nvol, sm, bk); });
             emu_threads(C::NT, [&](int t) { K::ypass(t, b, nvol, sm, dst, bk, (T)scl); });
             if (!lds.guards_ok() && g_emu_bad_block < 0) g_emu_bad_block = b;
-            if (nb >= nblocks) break;
-            b = nb;
-        }
-    }
+        });
     return hipSuccess;
 }
 
 template <class C>
 void emu_spipe(const cx<typename C::T>* src, cx<typename C::T>* dst, long nslab,
-               const Bank<typename C::T>* bk)
+               const Bank<typename C::T>* bk, unsigned* work)
 {
     using T = typename C::T;
     using K = SlabPipe<C>;
@@ -124,28 +160,25 @@ void emu_spipe(const cx<typename C::T>* src, cx<typename C::T>* dst, long nslab,
     const long nblocks = (nslab + C::SPW - 1) / C::SPW;
     const long grid = nblocks < kEmuPersistentGrid ? nblocks : kEmuPersistentGrid;
     EmuLds<T> lds(C::LDS);
-    std::vector<Box> R(C::NT);
-    for (long w = 0; w < grid; ++w) {
-        long b = w;
-        emu_threads(C::NT, [&](int t) { K::load_y(t, b, nslab, src, R[t].r); });
-        for (;;) {
+    std::vector<std::vector<Box>> R(grid, std::vector<Box>(C::NT));
+    emu_persistent(
+        nblocks, grid, nblocks >= 2 * grid ? work : nullptr,
+        [&](long w, long b) { emu_threads(C::NT, [&](int t) { K::load_y(t, b, nslab, src, R[w][t].r); }); },
+        [&](long w, long b, long nb) {
             lds.poison();
             cx<T>* sm = lds.data();
-            emu_threads(C::NT, [&](int t) { K::ypass(t, b, nslab, R[t].r, sm, bk); });
-            const long nb = b + grid;
-            if (nb < nblocks) emu_threads(C::NT, [&](int t) { K::load_y(t, nb, nslab, src, R[t].r); });
+            emu_threads(C::NT, [&](int t) { K::ypass(t, b, nslab, R[w][t].r, sm, bk); });
+            if (nb < nblocks)
+                emu_threads(C::NT, [&](int t) { K::load_y(t, nb, nslab, src, R[w][t].r); });
             emu_threads(C::NT, [&](int t) { K::zpass(t, b, nslab, sm, bk); });
             emu_threads(C::NT, [&](int t) { K::store(t, b, nslab, sm, dst); });
             if (!lds.guards_ok() && g_emu_bad_block < 0) g_emu_bad_block = b;
-            if (nb >= nblocks) break;
-            b = nb;
-        }
-    }
+        });
 }
 
 template <class C>
 hipError_t launch_pencil(const void* in, void* out, long nvol, const void* bank, double scl,
-                         hipStream_t)
+                         unsigned* work, hipStream_t)
 {
     using T = typename C::T;
     using K = Pencil<C>;
@@ -159,7 +192,7 @@ hipError_t launch_pencil(const void* in, void* out, long nvol, const void* bank,
         cx<T>* dst = (cx<T>*)out + v0 * C::V;
         const long nslab = nv * C::N0;
         const long ga = (nslab + C::SPW - 1) / C::SPW;
-        if constexpr (C::PIPE > 0) emu_spipe<C>(src, dst, nslab, bk);
+        if constexpr (C::PIPE > 0) emu_spipe<C>(src, dst, nslab, bk, work);
         else for (long b = 0; b < ga; ++b) {
             lds.poison();
             cx<T>* sm = lds.data();

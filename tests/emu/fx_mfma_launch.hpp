@@ -6,7 +6,7 @@
 namespace fx {
 
 template <class C, int SPLIT>
-hipError_t launch_mfma(const void*, void*, long, const void*, double, hipStream_t)
+hipError_t launch_mfma(const void*, void*, long, const void*, double, unsigned*, hipStream_t)
 {
     return hipErrorNotSupported;
 }

@@ -70,6 +70,32 @@ def test_every_variant_on_the_emulator(emu):
                     f"{name}: result depends on thread order"
 
 
+def test_pipelined_variants_claim_blocks_dynamically(emu):
+    """Persistent pipelined kernels on a batch of 8 blocks: the emulated grid of 3 workgroups
+    draws blocks from the launch's claim counter in interleaved order (both thread orders),
+    and the counter must come back at zero (emu_transform returns -3 otherwise)."""
+    n = emu.emu_kernel_count()
+    seen = 0
+    for i in range(n):
+        name = emu.emu_kernel_name(i).decode()
+        if not name.startswith("pipe_") or name.endswith("p0") or "p0_" in name:
+            continue
+        shape, prec, d, per_wg = parse(name)
+        dt = np.complex128 if prec == 0 else np.complex64
+        tol = 1e-13 if prec == 0 else 2e-6
+        batch = 7 * per_wg + 1
+        x = oracle.make_input(shape, batch, seed=i, dist="uniform", dtype=dt)
+        ref = oracle.fftn_numpy(x, sign=d)
+        for reverse in (0, 1):
+            out = np.full_like(x, np.nan)
+            rc = emu.emu_transform(name.encode(), x.ctypes.data, out.ctypes.data, batch, 1.0,
+                                   reverse)
+            assert rc == 0, f"{name}: rc {rc}"
+            assert oracle.rel_l2(out, ref) <= tol, name
+        seen += 1
+    assert seen >= 20
+
+
 def test_asan_sweep():
     """Every variant at 4 batch sizes with exactly-sized heap buffers under AddressSanitizer."""
     subprocess.run(["make", "-s", "-C", EMU, "emu_asan"], check=True)

@@ -12,7 +12,7 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(6, 6, 6), (8, 8, 8), (12, 12, 12), (17, 17, 17), (8, 8, 16)]
-BARS = {"bf16": 1e-4, "fp16": 1e-5}
+BARS = {"bf16": 2e-5, "fp16": 6e-7}  # observed 4-7e-6 and 1.1-1.9e-7
 
 
 @pytest.mark.parametrize("split", ["bf16", "fp16"])
@@ -25,11 +25,12 @@ def test_mfma_plans_match_oracle(gpu, split):
                 x = oracle.make_input(shape, batch, seed=7 + batch, dist="uniform",
                                       dtype=np.complex64)
                 plan = f.Plan(shape, batch, direction, f.C64, flags=flag)
-                assert plan.kernel.startswith("mfma_") and plan.kernel.endswith(split + "x3")
+                name = plan.kernel
+                assert name.startswith("mfma_") and name.endswith(split + "x3")
                 y = plan.run(x)
                 plan.close()
                 err = oracle.rel_l2(y, oracle.fftn_numpy(x.astype(np.complex128), sign=direction))
-                print(f"{plan.kernel} dir={direction} B={batch} rel_l2={err:.2e}", flush=True)
+                print(f"{name} dir={direction} B={batch} rel_l2={err:.2e}", flush=True)
                 assert np.all(np.isfinite(y))
                 assert err <= BARS[split], (shape, direction, batch, err)
 

@@ -15,7 +15,7 @@ TOL = {0: 1e-13, 1: 2e-6}
 # Matrix-core (DFT-as-matrix) complex64 variants: operands split into two half-precision
 # parts, three products accumulated in FP32.  Per pass the represented operands carry ~16
 # (bf16 x2) or ~22 (fp16 x2) significant bits; three passes of an N-term sum.
-TOL_MFMA = {"bf16x3": 1e-4, "fp16x3": 1e-5}
+TOL_MFMA = {"bf16x3": 2e-5, "fp16x3": 6e-7}
 
 
 def _tol(name, prec):
