@@ -12,6 +12,7 @@ const std::vector<KernelDesc>& registry()
         register_fused_17(v);
         register_pencil(v);
         register_mfma(v);
+        register_f17(v);
         return v;
     }();
     return r;
