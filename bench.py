@@ -8,6 +8,19 @@ HBM.  The timed region is K steps between HIP events on the launch stream, brack
 device synchronisations.  The other workloads (8^3 x 4096, 36^3 x 128, 24x24x48 x 64 and
 the 17^3 inverse) are timed the same way and reported under "secondary".
 
+"precision_variants": complex64 on the uneven batch set 6^3 x 65536, 8^3 x 32768,
+12^3 x 8192, 17^3 x 2048 -- as one persistent mixed launch, as per-geometry FFT plans, and
+as the DFT-as-matrix matrix-core variant (split bf16 / fp16, three products, FP32
+accumulate) -- with relative L2 and max error on 256 volumes per geometry against
+numpy.fft in float64, and for the matrix-core kernels the executed MFMA TFLOP/s against the
+2.5 PFLOP/s dense peak.
+
+"serving": the request server under Poisson arrivals (20,000 requests, the mixed stream of
+opt-fft_amd/serving.py, 64 GB budget, 16 launches in flight, 30 us batching window for the
+continuously batched server): the highest arrival rate whose p99 latency stays
+within 2 ms, for continuous batching and for the plain one-launch-per-request path, and
+the throughput ratio of the two at that cap.
+
 Prints ONE JSON line.  `python bench.py --gpus N` with N > 1 exits: this build runs on one
 GPU (batches shard by volume with no communication).
 """
@@ -51,6 +64,24 @@ def make_input(shape, batch, seed, dist):
         s = math.sqrt(0.5)
         re, im = rng.normal(0, s, full), rng.normal(0, s, full)
     return (re + 1j * im).astype(np.complex128)
+
+
+# Accuracy of a measured output, against numpy.fft in float64 (the reference's boundary check,
+# bench/geom/check.py:28-33); the unnormalised backward transform is V * ifftn.
+def _np_dft(x, sign=-1):
+    x = np.asarray(x, dtype=np.complex128)
+    if sign < 0:
+        return np.fft.fftn(x, axes=(-3, -2, -1))
+    return np.fft.ifftn(x, axes=(-3, -2, -1)) * (x.shape[-3] * x.shape[-2] * x.shape[-1])
+
+
+def _rel_l2(y, ref):
+    return float(np.linalg.norm((np.asarray(y, np.complex128) - ref).ravel()) /
+                 np.linalg.norm(ref.ravel()))
+
+
+def _rel_max(y, ref):
+    return float(np.max(np.abs(np.asarray(y, np.complex128) - ref)) / np.max(np.abs(ref)))
 
 
 def measure(pkg, wl, steps, warmup, check=True):
@@ -101,10 +132,9 @@ def measure(pkg, wl, steps, warmup, check=True):
     res["achieved_GBs"] = algo / (ms * 1e-3 / steps) / 1e9
     res["roofline_frac"] = res["achieved_GBs"] / PEAK_HBM_GBS
     if check:
-        import oracle
         # the last step's output pair: ((steps-1) % rot); its input is x
         y = outs[(steps - 1) % rot].download(np.complex128, x.shape)
-        res["rel_l2_vs_numpy"] = oracle.rel_l2(y, oracle.fftn_numpy(x, sign=wl["direction"]))
+        res["rel_l2_vs_numpy"] = _rel_l2(y, _np_dft(x, wl["direction"]))
     for b in ins + outs:
         b.close()
     stream.close()
@@ -164,6 +194,153 @@ def cpu_baseline(seconds=10.0):
                        f"repeated for {dt:.1f} s ({n} transforms), 1 thread")
 
 
+# Precision variants on uneven batches (complex64): one persistent mixed launch, the
+# per-geometry complex64 FFT plans, and the DFT-as-matrix variant on the matrix cores.
+C4_ITEMS = [((6, 6, 6), 65536), ((8, 8, 8), 32768), ((12, 12, 12), 8192), ((17, 17, 17), 2048)]
+C4_CHECK = 256          # volumes per item compared with numpy.fft in float64
+PEAK_MFMA_TFLOPS = 2500.0   # dense bf16/fp16 matrix peak of one MI355X
+
+
+def _uniform_c64(rng, shape, batch):
+    full = (batch,) + tuple(shape)
+    return (rng.uniform(-1, 1, full) + 1j * rng.uniform(-1, 1, full)).astype(np.complex64)
+
+
+def mfma_flops(kernel, shape, nvol):
+    """MFMA flops the matrix-core kernel executes for nvol volumes: per block of VPW volumes
+    and per axis pass of length N, ceil(lines/16) tiles of T x T x 3 16x16x16 products
+    (T = ceil(2N/16), three products per tile for the compensated split)."""
+    import re
+    vpw = int(re.search(r"_v(\d+)t", kernel).group(1))
+    V = shape[0] * shape[1] * shape[2]
+    per_block = 0
+    for n in shape:
+        T = (2 * n + 15) // 16
+        per_block += -(-(vpw * V // n) // 16) * T * T * 3
+    full, part = divmod(nvol, vpw)
+    insts = full * per_block
+    if part:
+        for n in shape:
+            T = (2 * n + 15) // 16
+            insts += -(-(part * V // n) // 16) * T * T * 3
+    return insts * 2.0 * 16 * 16 * 16
+
+
+def precision_mix(pkg, steps=20):
+    f, h = pkg.fft3d, pkg.hip
+    rng = np.random.default_rng(44)
+    xs = [_uniform_c64(rng, sh, b) for sh, b in C4_ITEMS]
+    nbytes = sum(x.nbytes for x in xs)
+    sets = 2
+    ins = [[h.DeviceBuffer(x.nbytes) for x in xs] for _ in range(sets)]
+    outs = [[h.DeviceBuffer(x.nbytes) for x in xs] for _ in range(sets)]
+    for s_ in range(sets):
+        for b, x in zip(ins[s_], xs):
+            b.upload(x)
+    stream = h.Stream()
+    e0, e1 = h.Event(), h.Event()
+
+    def timed(launch):
+        for i in range(3):
+            launch(i % sets)
+        h.synchronize()
+        e0.record(stream)
+        for i in range(steps):
+            launch(i % sets)
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / steps * 1e-3
+
+    def errors(bufs):
+        out = {}
+        for (sh, b), x, ob in zip(C4_ITEMS, xs, bufs):
+            y = ob.download(np.complex64, x.shape)[:C4_CHECK]
+            ref = _np_dft(x[:C4_CHECK])
+            out[f"L{sh[0]}"] = dict(rel_l2=_rel_l2(y, ref), rel_max=_rel_max(y, ref))
+        return out
+
+    res = {"items": [dict(shape=list(sh), batch=b) for sh, b in C4_ITEMS],
+           "bytes_per_launch": 2 * nbytes, "error_sample_volumes": C4_CHECK}
+    mp = f.MultiPlan([(sh, b, -1) for sh, b in C4_ITEMS], f.C64)
+    t = timed(lambda s_: mp.execute([b.addr for b in ins[s_]], [b.addr for b in outs[s_]],
+                                    stream=stream))
+    h.synchronize()
+    res["one_launch_fft_c64"] = dict(kernel=mp.info()[0], us=t * 1e6, GBs=2 * nbytes / t / 1e9,
+                                     errors=errors(outs[0]))
+    mp.close()
+    for tag, flag in (("fft_c64", 0), ("mfma_bf16x3", f.MFMA_BF16), ("mfma_fp16x3", f.MFMA_FP16)):
+        per = {}
+        for k, ((sh, b), x) in enumerate(zip(C4_ITEMS, xs)):
+            p = f.Plan(sh, b, f.FORWARD, f.C64, flags=flag)
+            tk = timed(lambda s_: p.execute(ins[s_][k].addr, outs[s_][k].addr, stream=stream))
+            h.synchronize()
+            d = dict(kernel=p.kernel, us=tk * 1e6, GBs=2 * x.nbytes / tk / 1e9)
+            y = outs[(steps - 1) % sets][k].download(np.complex64, x.shape)[:C4_CHECK]
+            ref = _np_dft(x[:C4_CHECK])
+            d["rel_l2"] = _rel_l2(y, ref)
+            d["rel_max"] = _rel_max(y, ref)
+            if flag:
+                fl = mfma_flops(p.kernel, sh, b)
+                d["mfma_tflops"] = fl / tk / 1e12
+                d["mfma_util"] = d["mfma_tflops"] / PEAK_MFMA_TFLOPS
+            per[f"L{sh[0]}"] = d
+            p.close()
+        res[tag] = per
+    for lst in ins + outs:
+        for b in lst:
+            b.close()
+    return res
+
+
+def serving(pkg, n=20000, p99_cap_ms=2.0, streams=16, window_us=30.0):
+    """Load test of the request server: the highest Poisson arrival rate whose p99 latency
+    stays within p99_cap_ms, for continuous batching and for the plain path."""
+    f, sv = pkg.fft3d, pkg.serving
+    mix = sv.draw_mix(n, seed=0)
+    pools = sv.InputPools(seed=0)
+    reqs = sv.make_requests(mix, pools, seed=1)
+    req_bytes = sum(2 * r.batch * r.n0 * r.n1 * r.n2 * (16 if r.precision == 0 else 8)
+                    for r in reqs) / n
+    out = {"requests": n, "mean_io_bytes_per_request": req_bytes, "p99_cap_ms": p99_cap_ms,
+           "budget_bytes": 64 << 30, "streams": streams, "window_us": window_us}
+    for mode in ("plain", "batched"):
+        srv = f.Server(budget_bytes=64 << 30, streams=streams, batched=(mode == "batched"),
+                       window_us=window_us if mode == "batched" else 0.0)
+        ok, bad, pts = None, None, []
+        rate = 4000.0
+        while rate < 2e6:
+            pt = sv.load_point(srv, reqs, rate)
+            pts.append(pt)
+            if pt["p99_ms"] <= p99_cap_ms:
+                ok = pt
+                rate *= 1.5
+            else:
+                bad = pt
+                break
+        if ok and bad:
+            lo, hi = ok["rate"], bad["rate"]
+            for _ in range(4):
+                mid = 0.5 * (lo + hi)
+                pt = sv.load_point(srv, reqs, mid)
+                pts.append(pt)
+                if pt["p99_ms"] <= p99_cap_ms:
+                    ok, lo = pt, mid
+                else:
+                    hi = mid
+        st = srv.stats()
+        srv.close()
+        out[mode] = dict(max_rate_req_s=ok["rate"] if ok else 0.0,
+                         at_max=ok, stats=st,
+                         points=[{k: round(v, 4) for k, v in p.items()} for p in pts])
+        print(f"# serving {mode}: max rate {out[mode]['max_rate_req_s']:.0f}/s", file=sys.stderr,
+              flush=True)
+    pools.close()
+    pl, ba = out["plain"]["at_max"], out["batched"]["at_max"]
+    if pl and ba:
+        out["throughput_ratio_at_p99_cap"] = ba["transforms_per_s"] / pl["transforms_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +348,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; batches of independent volumes "
@@ -195,6 +374,10 @@ def main():
         for wl in SECONDARY:
             secondary[wl["name"]] = measure(pkg, wl, max(10, a.steps // 2), a.warmup)
             print(f"# {wl['name']}: {secondary[wl['name']]}", file=sys.stderr, flush=True)
+    precision = None if a.no_precision else precision_mix(pkg)
+    if precision:
+        print(f"# precision variants: {precision}", file=sys.stderr, flush=True)
+    serve = None if a.no_serving else serving(pkg)
     V = 17 ** 3
     line = {
         "metric": "batched transforms/s + % HBM roofline at L=8/17/36 fp64; serving p99 latency",
@@ -222,6 +405,8 @@ def main():
                      "algorithmic_bytes_per_launch": head["algorithmic_bytes_per_execute"]},
         "cpu_baseline": cpu,
         "secondary": secondary,
+        "precision_variants": precision,
+        "serving": serve,
     }
     print(json.dumps(line), flush=True)
 

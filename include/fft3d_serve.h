@@ -33,9 +33,12 @@ typedef struct fft3d_srv fft3d_srv;
 typedef struct fft3d_srv_config {
     size_t budget_bytes; /* admission cap on in+out bytes of unfinished requests; also the
                             size of the output arena */
-    int streams;         /* launches kept in flight (1..8) */
+    int streams;         /* launches kept in flight (1..16) */
     int batched;         /* 1 continuous batching, 0 plain one-launch-per-request */
     int max_items;       /* requests per mixed launch (1..16) */
+    double window_us;    /* batched mode: hold a partial mixed batch open for up to this long
+                            after its oldest request was admitted, so later arrivals can join
+                            (0: launch as soon as a stream is free; at most 1e6) */
 } fft3d_srv_config;
 
 typedef struct fft3d_srv_request {

@@ -34,7 +34,7 @@ double steady_now()
         .count();
 }
 
-constexpr int kMaxStreams = 8;
+constexpr int kMaxStreams = 16;
 constexpr int kMaxBatch = 512;
 constexpr size_t kAlign = 256;
 
@@ -91,7 +91,7 @@ struct Req {
     size_t out_bytes;
     void* out;         // where the result goes
     bool arena_out;
-    double t_arrive = 0, t_done = 0;
+    double t_arrive = 0, t_done = 0, t_ready = 0;
     std::atomic<int> done{0};
 };
 
@@ -307,6 +307,7 @@ static void admit(fft3d_srv* s)
         s->inflight_bytes += q->io_bytes;
         s->st.peak_inflight_bytes = std::max(s->st.peak_inflight_bytes, s->inflight_bytes);
         s->pending.pop_front();
+        q->t_ready = steady_now();
         s->ready.push_back(q);
     }
 }
@@ -322,6 +323,23 @@ static void dispatch(fft3d_srv* s)
             if (!launch_plan(s, head, stream, !s->cfg.batched)) fail_reqs(s, {head});
             if (!s->cfg.batched) retire(s);
             continue;
+        }
+        // A partial mixed batch stays open for up to window_us after its oldest request was
+        // admitted; meanwhile plan-path requests behind it still launch.
+        if (s->cfg.window_us > 0) {
+            int n = 0;
+            for (Req* q : s->ready)
+                if (q->code >= 0 && q->r.precision == head->r.precision && ++n >= s->cfg.max_items)
+                    break;
+            if (n < s->cfg.max_items && steady_now() - head->t_ready < s->cfg.window_us * 1e-6) {
+                auto it = std::find_if(s->ready.begin(), s->ready.end(),
+                                       [](const Req* q) { return q->code < 0; });
+                if (it == s->ready.end()) return;
+                Req* q = *it;
+                s->ready.erase(it);
+                if (!launch_plan(s, q, stream, false)) fail_reqs(s, {q});
+                continue;
+            }
         }
         // Pack ready fused requests of the head's precision, oldest first.
         std::vector<Req*> pack;
@@ -430,7 +448,8 @@ extern "C" double fft3d_srv_now(void) { return steady_now(); }
 extern "C" fft3d_srv* fft3d_srv_create(const fft3d_srv_config* cfg)
 {
     if (!cfg || cfg->budget_bytes < (1u << 20) || cfg->streams < 1 || cfg->streams > kMaxStreams ||
-        cfg->max_items < 1 || cfg->max_items > kMultiMaxItems) {
+        cfg->max_items < 1 || cfg->max_items > kMultiMaxItems || !(cfg->window_us >= 0) ||
+        cfg->window_us > 1e6) {
         fail(FFT3D_MX_EINVAL, "fft3d_srv_create: bad configuration");
         return nullptr;
     }

@@ -48,7 +48,8 @@ EXPORTED = [
 
 class SrvConfig(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_size_t), ("streams", ctypes.c_int),
-                ("batched", ctypes.c_int), ("max_items", ctypes.c_int)]
+                ("batched", ctypes.c_int), ("max_items", ctypes.c_int),
+                ("window_us", ctypes.c_double)]
 
 
 class SrvRequest(ctypes.Structure):
@@ -377,8 +378,10 @@ class Server:
     """fft3d_srv: continuous batching (batched=True) or the plain one-launch-per-request
     path (batched=False)."""
 
-    def __init__(self, budget_bytes=64 << 30, streams=4, batched=True, max_items=16):
-        cfg = SrvConfig(int(budget_bytes), int(streams), 1 if batched else 0, int(max_items))
+    def __init__(self, budget_bytes=64 << 30, streams=8, batched=True, max_items=16,
+                 window_us=0.0):
+        cfg = SrvConfig(int(budget_bytes), int(streams), 1 if batched else 0, int(max_items),
+                        float(window_us))
         h = load().fft3d_srv_create(ctypes.byref(cfg))
         if not h:
             raise RuntimeError(last_error())

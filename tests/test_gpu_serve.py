@@ -40,11 +40,12 @@ def _check(gpu, mix, xs, outs):
     return res
 
 
-def test_mix_matches_oracle_and_alone(gpu):
+@pytest.mark.parametrize("streams,window_us", [(4, 0.0), (16, 30.0)])
+def test_mix_matches_oracle_and_alone(gpu, streams, window_us):
     f = gpu.fft3d
     mix = _mix(gpu, 120, 24, seed=3)
     xs, ins, outs = _setup(gpu, mix, 100)
-    srv = f.Server(budget_bytes=1 << 30, streams=4, batched=True)
+    srv = f.Server(budget_bytes=1 << 30, streams=streams, batched=True, window_us=window_us)
     reqs = [f.SrvRequest(*shape, b, d, p, bi.addr, bo.addr)
             for (shape, b, d, p), bi, bo in zip(mix, ins, outs)]
     # a burst: everything arrives within 200 us, so launches pack several requests
@@ -80,6 +81,14 @@ def test_plain_path_matches_oracle(gpu):
     srv.close()
     for b in ins + outs:
         b.close()
+
+
+def test_bad_configuration_is_rejected(gpu):
+    f = gpu.fft3d
+    for kw in (dict(streams=0), dict(streams=17), dict(max_items=0), dict(max_items=17),
+               dict(window_us=-1.0), dict(window_us=2e6), dict(budget_bytes=1 << 10)):
+        with pytest.raises(RuntimeError):
+            f.Server(**kw)
 
 
 def test_admission_respects_budget(gpu):
