@@ -98,19 +98,26 @@ static std::string wisdom_key(int n0, int n1, int n2, int prec, int dir, long ba
     return buf;
 }
 
-// Time one candidate: 2 warm launches, then the best of 3 groups of 5.  Seconds per
-// launch, or -1 if it failed to launch.
-static double trial(const KernelDesc& k, const void* din, void* dout, long nvol,
-                    const void* bank, unsigned* work, hipStream_t s, hipEvent_t e0,
+// Time one candidate: 2 warm launches, then the best of 3 groups of 5.  Successive launches
+// use successive windows of the race buffers, so with enough windows no launch finds its
+// data in the 256 MB Infinity Cache and the race ranks kernels as they run from HBM.
+// Seconds per launch, or -1 if it failed to launch.
+static double trial(const KernelDesc& k, const char* din, char* dout, size_t wbytes, int nwin,
+                    long nvol, const void* bank, unsigned* work, hipStream_t s, hipEvent_t e0,
                     hipEvent_t e1)
 {
+    int w = 0;
+    auto launch = [&]() {
+        const size_t off = (size_t)(w++ % nwin) * wbytes;
+        return k.launch(din + off, dout + off, nvol, bank, 1.0, work, s);
+    };
     for (int i = 0; i < 2; ++i)
-        if (k.launch(din, dout, nvol, bank, 1.0, work, s) != hipSuccess) return -1.0;
+        if (launch() != hipSuccess) return -1.0;
     double best = 1e30;
     for (int g = 0; g < 3; ++g) {
         if (hipEventRecord(e0, s) != hipSuccess) return -1.0;
         for (int i = 0; i < 5; ++i)
-            if (k.launch(din, dout, nvol, bank, 1.0, work, s) != hipSuccess) return -1.0;
+            if (launch() != hipSuccess) return -1.0;
         if (hipEventRecord(e1, s) != hipSuccess) return -1.0;
         if (hipEventSynchronize(e1) != hipSuccess) return -1.0;
         float ms = 0.f;
@@ -126,19 +133,29 @@ static const KernelDesc* race(const std::vector<const KernelDesc*>& cands, int n
                               double* best_s)
 {
     const size_t vol_bytes = (size_t)n0 * n1 * n2 * elem_bytes(prec);
-    // Cap the race buffers at ~256 MiB each so planning stays bounded; at least one
-    // volume, at most the planned batch.
+    // Each trial launch transforms rb volumes: ~256 MiB of input so planning stays bounded,
+    // at least one volume, at most the planned batch.
     long rb = (long)((256ull << 20) / vol_bytes);
     if (rb < 1) rb = 1;
     if (rb > batch) rb = batch;
+    const size_t wbytes = (size_t)rb * vol_bytes;
+    // Windows per buffer: enough for ~768 MiB, at most 8; fewer if memory is short.
+    int nwin = (int)(((768ull << 20) + wbytes - 1) / wbytes);
+    if (nwin > 8) nwin = 8;
     void *din = nullptr, *dout = nullptr;
-    if (hipMalloc(&din, rb * vol_bytes) != hipSuccess) return cands[0];
-    if (hipMalloc(&dout, rb * vol_bytes) != hipSuccess) {
+    for (; nwin >= 1; nwin = nwin > 1 ? nwin / 2 : 0) {
+        if (hipMalloc(&din, nwin * wbytes) != hipSuccess) {
+            din = nullptr;
+            continue;
+        }
+        if (hipMalloc(&dout, nwin * wbytes) == hipSuccess) break;
         (void)hipFree(din);
-        return cands[0];
+        din = dout = nullptr;
     }
+    (void)hipGetLastError();
+    if (!din || !dout) return cands[0];
     // 0x3F80_0000 per 32-bit word: 1.0f for complex64, ~0.0078 per double for complex128.
-    (void)hipMemsetD32((hipDeviceptr_t)din, 0x3F800000u, rb * vol_bytes / 4);
+    (void)hipMemsetD32((hipDeviceptr_t)din, 0x3F800000u, nwin * wbytes / 4);
     hipStream_t s;
     hipEvent_t e0, e1;
     const KernelDesc* win = cands[0];
@@ -147,7 +164,8 @@ static const KernelDesc* race(const std::vector<const KernelDesc*>& cands, int n
         if (hipEventCreate(&e0) == hipSuccess) {
             if (hipEventCreate(&e1) == hipSuccess) {
                 for (const KernelDesc* k : cands) {
-                    const double t = trial(*k, din, dout, rb, bank, work, s, e0, e1);
+                    const double t = trial(*k, (const char*)din, (char*)dout, wbytes, nwin,
+                                           rb, bank, work, s, e0, e1);
                     if (t > 0 && t < wt) {
                         wt = t;
                         win = k;
