@@ -37,9 +37,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch of the headline kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE
-# (profiles/), or None until measured.
-TRAFFIC = None
+# HBM bytes per launch of the headline kernel (pipe_17x17x17_c128_fwd_v1t320p1_fold, the
+# race's pick) from rocprofv3 FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on the
+# runtime's copy of a known size: 39,441.3 KiB x 2 + 78,654.8 KiB
+# (profiles/k_fpipe_17x17x17_c128_fwd_b1024.md).
+TRAFFIC = int((39441.3 * 2 + 78654.8) * 1024)
+TRAFFIC_KERNEL = "pipe_17x17x17_c128_fwd_v1t320p1_fold"
 ROTATE_BYTES = 2 << 30
 
 HEADLINE = dict(name="L17_c128_fwd_b1024", shape=(17, 17, 17), batch=1024, dist="uniform",
@@ -400,7 +403,8 @@ def main():
         "gflops_nominal": 5.0 * V * math.log2(V) * head["transforms_per_s"] / 1e9,
         "rel_l2_vs_numpy": head["rel_l2_vs_numpy"],
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": head["roofline_frac"], "traffic": TRAFFIC,
+                     "unit": "GB/s", "frac": head["roofline_frac"],
+                     "traffic": TRAFFIC if head["kernel"] == TRAFFIC_KERNEL else None,
                      "kernel_us_mean": head["ms_per_execute"] * 1e3,
                      "algorithmic_bytes_per_launch": head["algorithmic_bytes_per_execute"]},
         "cpu_baseline": cpu,
