@@ -5,13 +5,14 @@ namespace fx {
 
 template <typename T, int PRESET>
 hipError_t launch_multi(const void* args, const int2* tasks, int ntasks, const void* bank,
-                        hipStream_t stream)
+                        unsigned* work, hipStream_t stream)
 {
     using M = MultiCfg<T, PRESET>;
     if (ntasks <= 0) return hipSuccess;
     const long grid = persistent_grid(k_multi<T, PRESET>, M::NT, 8, ntasks);
     hipLaunchKernelGGL((k_multi<T, PRESET>), dim3((unsigned)grid), dim3(M::NT), 0, stream,
-                       *(const MultiArgs<T>*)args, tasks, ntasks, (const Bank<T>*)bank);
+                       *(const MultiArgs<T>*)args, tasks, ntasks, (const Bank<T>*)bank,
+                       claim_counter(work, grid, ntasks));
     return hipGetLastError();
 }
 
@@ -20,7 +21,7 @@ hipError_t launch_multi_dev(const void* d, const void* bank, int grid, hipStream
 {
     using M = MultiCfg<T, PRESET>;
     hipLaunchKernelGGL((k_multi_dev<T, PRESET>), dim3((unsigned)grid), dim3(M::NT), 0, stream,
-                       (const MultiDescriptor<T>*)d, (const Bank<T>*)bank);
+                       (MultiDescriptor<T>*)d, (const Bank<T>*)bank);
     return hipGetLastError();
 }
 

@@ -3,7 +3,7 @@
 //
 // The host cuts every item into blocks (VPW volumes, the tile the preset gives that
 // geometry), sorts the blocks heaviest first into a task list, and launches one grid of
-// resident workgroups that walk the list round-robin.  A task runs the pipelined fused
+// resident workgroups that take tasks from a claim counter (claim_block).  A task runs the pipelined fused
 // pass structure of its geometry (x from HBM into VGPRs, z in LDS, y from LDS to HBM) out
 // of one LDS buffer sized for the largest block.  The backward direction reuses the forward
 // code: conj(DFT(conj(x))) is the unnormalised backward DFT, and both conjugations are sign
@@ -112,59 +112,73 @@ FX_HD void multi_task_phase(int ph, int tid, int2 task, const MultiArgs<T>& a, c
     }
 }
 
+// Walk the task list: the first task is blockIdx.x, the next ones are claimed from `work`
+// (NULL: a fixed stride of gridDim.x).  The claim is issued before phase 0 and published to
+// the workgroup at the first barrier.
 template <typename T, int PRESET>
-__global__ void __launch_bounds__((MultiCfg<T, PRESET>::NT))
-    __attribute__((amdgpu_waves_per_eu(MultiCfg<T, PRESET>::WPE, 8)))
-    k_multi(const MultiArgs<T> a, const int2* __restrict__ tasks, int ntasks,
-            const Bank<T>* __restrict__ bk)
+__device__ __forceinline__ void multi_walk(const MultiArgs<T>& a, const int2* __restrict__ tasks,
+                                           int ntasks, const Bank<T>* __restrict__ bk,
+                                           unsigned* work, cx<T>* sm, long* s_next)
 {
-    using M = MultiCfg<T, PRESET>;
-    __shared__ cx<T> sm[M::LDS];
     const int tid = threadIdx.x;
-    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+    long t = blockIdx.x;
+    while (t < ntasks) {
+        long claim = 0;
+        if (tid == 0) claim = claim_block(work, ntasks, t);
         const int2 task = tasks[t];
         multi_task_phase<T, PRESET>(0, tid, task, a, sm, bk);
+        if (tid == 0) *s_next = claim;
         __syncthreads();
+        const long nt = *s_next;
         multi_task_phase<T, PRESET>(1, tid, task, a, sm, bk);
         __syncthreads();
         multi_task_phase<T, PRESET>(2, tid, task, a, sm, bk);
         __syncthreads();
+        t = nt;
     }
+}
+
+template <typename T, int PRESET>
+__global__ void __launch_bounds__((MultiCfg<T, PRESET>::NT))
+    __attribute__((amdgpu_waves_per_eu(MultiCfg<T, PRESET>::WPE, 8)))
+    k_multi(const MultiArgs<T> a, const int2* __restrict__ tasks, int ntasks,
+            const Bank<T>* __restrict__ bk, unsigned* __restrict__ work)
+{
+    using M = MultiCfg<T, PRESET>;
+    __shared__ cx<T> sm[M::LDS];
+    __shared__ long s_next;
+    multi_walk<T, PRESET>(a, tasks, ntasks, bk, work, sm, &s_next);
 }
 
 // Same kernel with the items, the task count and the task list read from device memory, so a
 // launch can be captured once in a hipGraph and replayed with new descriptors (serving).
+// `work` is the launch's claim counter: zero in every descriptor the host writes, and left
+// at zero by the launch (claim_block).
 template <typename T>
 struct MultiDescriptor {
     MultiArgs<T> args;
     int ntasks;
-    int pad_;
+    unsigned work;
     int2 tasks[1];  // ntasks entries follow
 };
 
 template <typename T, int PRESET>
 __global__ void __launch_bounds__((MultiCfg<T, PRESET>::NT))
     __attribute__((amdgpu_waves_per_eu(MultiCfg<T, PRESET>::WPE, 8)))
-    k_multi_dev(const MultiDescriptor<T>* __restrict__ d, const Bank<T>* __restrict__ bk)
+    k_multi_dev(MultiDescriptor<T>* __restrict__ d, const Bank<T>* __restrict__ bk)
 {
     using M = MultiCfg<T, PRESET>;
     __shared__ cx<T> sm[M::LDS];
-    const int tid = threadIdx.x;
+    __shared__ long s_next;
     const int ntasks = d->ntasks;
-    for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
-        const int2 task = d->tasks[t];
-        multi_task_phase<T, PRESET>(0, tid, task, d->args, sm, bk);
-        __syncthreads();
-        multi_task_phase<T, PRESET>(1, tid, task, d->args, sm, bk);
-        __syncthreads();
-        multi_task_phase<T, PRESET>(2, tid, task, d->args, sm, bk);
-        __syncthreads();
-    }
+    unsigned* work = ntasks >= 2L * gridDim.x ? &d->work : nullptr;
+    multi_walk<T, PRESET>(d->args, d->tasks, ntasks, bk, work, sm, &s_next);
 }
 
 // Registry entry of a multi kernel.
+// `work`: claim counter as for LaunchFn (registry.hpp).
 typedef hipError_t (*MultiLaunchFn)(const void* args, const int2* tasks, int ntasks,
-                                    const void* bank, hipStream_t stream);
+                                    const void* bank, unsigned* work, hipStream_t stream);
 // Launch of the descriptor-reading form on `grid` workgroups.
 typedef hipError_t (*MultiDevLaunchFn)(const void* descriptor, const void* bank, int grid,
                                        hipStream_t stream);

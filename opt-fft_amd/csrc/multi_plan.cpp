@@ -1,4 +1,5 @@
 // Host side of the mixed-batch persistent launch (include/fft3d_mi355x.h, multi.hpp).
+#include <atomic>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,8 +25,20 @@ struct fft3d_mx_multi {
     int2* tasks;   // device
     int ntasks;
     void* bank;    // device
+    unsigned* work;  // device: kMultiWorkSlots claim counters, one 64 B line each
+    std::atomic<unsigned> work_next;
     int device;
 };
+
+// Launches of one mixed plan take successive claim counters, so up to this many may be in
+// flight at once.
+constexpr int kMultiWorkSlots = 64;
+
+static unsigned* next_work(fft3d_mx_multi* m)
+{
+    const unsigned i = m->work_next.fetch_add(1u, std::memory_order_relaxed) % kMultiWorkSlots;
+    return m->work + (size_t)i * 16;
+}
 
 template <typename T>
 static void fill_args(const fft3d_mx_multi* m, const void* const* in, void* const* out,
@@ -42,18 +55,18 @@ static void fill_args(const fft3d_mx_multi* m, const void* const* in, void* cons
     }
 }
 
-static hipError_t multi_launch(const fft3d_mx_multi* m, const MultiDesc* k, const int2* tasks,
+static hipError_t multi_launch(fft3d_mx_multi* m, const MultiDesc* k, const int2* tasks,
                                int ntasks, const void* const* in, void* const* out,
                                hipStream_t s)
 {
     if (m->prec == kC128) {
         MultiArgs<double> a;
         fill_args(m, in, out, a);
-        return k->launch(&a, tasks, ntasks, m->bank, s);
+        return k->launch(&a, tasks, ntasks, m->bank, next_work(m), s);
     }
     MultiArgs<float> a;
     fill_args(m, in, out, a);
-    return k->launch(&a, tasks, ntasks, m->bank, s);
+    return k->launch(&a, tasks, ntasks, m->bank, next_work(m), s);
 }
 
 static int upload_tasks(const std::vector<int2>& t, int2** dev)
@@ -182,9 +195,13 @@ extern "C" fft3d_mx_multi* fft3d_mx_multi_create(const fft3d_mx_item* items, int
     bank_host_image(precision, img);
     hipError_t e = hipMalloc(&m->bank, img.size());
     if (e == hipSuccess) e = hipMemcpy(m->bank, img.data(), img.size(), hipMemcpyHostToDevice);
+    const size_t work_bytes = sizeof(unsigned) * 16 * kMultiWorkSlots;
+    if (e == hipSuccess) e = hipMalloc((void**)&m->work, work_bytes);
+    if (e == hipSuccess) e = hipMemset(m->work, 0, work_bytes);
     if (e != hipSuccess) {
         fail(FFT3D_MX_ENOMEM, "fft3d_mx_multi_create: bank: %s", hipGetErrorString(e));
         (void)hipFree(m->bank);
+        (void)hipFree(m->work);
         delete m;
         return nullptr;
     }
@@ -196,6 +213,7 @@ extern "C" fft3d_mx_multi* fft3d_mx_multi_create(const fft3d_mx_item* items, int
     if (cands.empty()) {
         fail(FFT3D_MX_EUNSUPPORTED, "fft3d_mx_multi_create: no mixed kernel named %s", forced);
         (void)hipFree(m->bank);
+        (void)hipFree(m->work);
         delete m;
         return nullptr;
     }
@@ -205,6 +223,7 @@ extern "C" fft3d_mx_multi* fft3d_mx_multi_create(const fft3d_mx_item* items, int
     m->ntasks = (int)t.size();
     if (upload_tasks(t, &m->tasks) != FFT3D_MX_OK) {
         (void)hipFree(m->bank);
+        (void)hipFree(m->work);
         delete m;
         return nullptr;
     }
@@ -245,5 +264,6 @@ extern "C" void fft3d_mx_multi_destroy(fft3d_mx_multi* m)
     if (!m) return;
     (void)hipFree(m->tasks);
     (void)hipFree(m->bank);
+    (void)hipFree(m->work);
     delete m;
 }

@@ -190,6 +190,7 @@ static int launch_mixed(fft3d_srv* s, std::vector<Req*>& reqs, int stream)
             it.scale = (T)1;
         }
         desc->ntasks = (int)tasks.size();
+        desc->work = 0u;
         std::memcpy(desc->tasks, tasks.data(), tasks.size() * sizeof(int2));
         return offsetof(std::remove_reference_t<decltype(*desc)>, tasks) + tasks.size() * sizeof(int2);
     };

@@ -48,8 +48,8 @@ extern "C" const char* emu_multi_name(int i)
 }
 
 // Run mixed kernel `name` over nitems items (shape[3*i..], batch[i], dir[i], scale[i]) on
-// host buffers.  Returns 0, -1 for an unknown name or geometry, -1000 - task on an LDS
-// guard violation.
+// host buffers.  Returns 0, -1 for an unknown name or geometry, -3 when the claim counter is
+// not back at zero, -1000 - task on an LDS guard violation.
 template <typename T>
 static int emu_multi_t(const MultiDesc& d, int n, const int* shape, const int* batch,
                        const int* dir, const double* scale, const void* const* in,
@@ -69,8 +69,10 @@ static int emu_multi_t(const MultiDesc& d, int n, const int* shape, const int* b
     bank_host_image(d.prec, img);
     g_emu_reverse = 0;
     g_emu_bad_block = -1;
-    (void)d.launch(&a, tasks.data(), (int)tasks.size(), img.data(), nullptr);
-    return g_emu_bad_block >= 0 ? (int)(-1000 - g_emu_bad_block) : 0;
+    unsigned work = 0;  // claim counter; the launch must hand it back at zero
+    (void)d.launch(&a, tasks.data(), (int)tasks.size(), img.data(), &work, nullptr);
+    if (g_emu_bad_block >= 0) return (int)(-1000 - g_emu_bad_block);
+    return work == 0 ? 0 : -3;
 }
 
 extern "C" int emu_multi(const char* name, int n, const int* shape, const int* batch,

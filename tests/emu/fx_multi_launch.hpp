@@ -7,29 +7,31 @@ namespace fx {
 
 template <typename T, int PRESET>
 hipError_t launch_multi(const void* args, const int2* tasks, int ntasks, const void* bank,
-                        hipStream_t)
+                        unsigned* work, hipStream_t)
 {
     using M = MultiCfg<T, PRESET>;
     const MultiArgs<T>& a = *(const MultiArgs<T>*)args;
     const Bank<T>* bk = (const Bank<T>*)bank;
+    if (ntasks <= 0) return hipSuccess;
     EmuLds<T> lds(M::LDS);
-    const int grid = ntasks < 3 ? ntasks : 3;
-    for (int w = 0; w < grid; ++w)
-        for (int t = w; t < ntasks; t += grid) {
+    const long grid = ntasks < kEmuPersistentGrid ? ntasks : kEmuPersistentGrid;
+    emu_persistent(
+        ntasks, grid, ntasks >= 2 * grid ? work : nullptr, [](long, long) {},
+        [&](long, long t, long) {
             lds.poison();
             cx<T>* sm = lds.data();
             for (int ph = 0; ph < 3; ++ph)
                 emu_threads(M::NT, [&](int tid) { multi_task_phase<T, PRESET>(ph, tid, tasks[t], a, sm, bk); });
             if (!lds.guards_ok() && g_emu_bad_block < 0) g_emu_bad_block = t;
-        }
+        });
     return hipSuccess;
 }
 
 template <typename T, int PRESET>
 hipError_t launch_multi_dev(const void* d, const void* bank, int, hipStream_t)
 {
-    const MultiDescriptor<T>* md = (const MultiDescriptor<T>*)d;
-    return launch_multi<T, PRESET>(&md->args, md->tasks, md->ntasks, bank, nullptr);
+    MultiDescriptor<T>* md = (MultiDescriptor<T>*)d;
+    return launch_multi<T, PRESET>(&md->args, md->tasks, md->ntasks, bank, &md->work, nullptr);
 }
 
 template <typename T, int PRESET>
