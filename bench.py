@@ -9,9 +9,10 @@ device synchronisations.  The other workloads (8^3 x 4096, 36^3 x 128, 24x24x48 
 the 17^3 inverse) are timed the same way and reported under "secondary".
 
 "precision_variants": complex64 on the uneven batch set 6^3 x 65536, 8^3 x 32768,
-12^3 x 8192, 17^3 x 2048 -- as one persistent mixed launch, as per-geometry FFT plans, and
-as the DFT-as-matrix matrix-core variant (split bf16 / fp16, three products, FP32
-accumulate) -- with relative L2 and max error on 256 volumes per geometry against
+12^3 x 8192, 17^3 x 2048 -- as one persistent mixed launch (FFT, and the DFT-as-matrix
+matrix-core form), as per-geometry FFT plans, and as per-geometry matrix-core plans (split
+bf16 / fp16, three products, FP32 accumulate) -- with relative L2 and max error on 256
+volumes per geometry against
 numpy.fft in float64, and for the matrix-core kernels the executed MFMA TFLOP/s against the
 2.5 PFLOP/s dense peak.
 
@@ -209,12 +210,15 @@ def _uniform_c64(rng, shape, batch):
     return (rng.uniform(-1, 1, full) + 1j * rng.uniform(-1, 1, full)).astype(np.complex64)
 
 
-def mfma_flops(kernel, shape, nvol):
+# Volumes per block of the matrix-core kernels (standalone and mixed launch alike).
+MFMA_VPW = {(6, 6, 6): 16, (8, 8, 8): 8, (12, 12, 12): 2, (17, 17, 17): 1, (8, 8, 16): 4}
+
+
+def mfma_flops(shape, nvol):
     """MFMA flops the matrix-core kernel executes for nvol volumes: per block of VPW volumes
     and per axis pass of length N, ceil(lines/16) tiles of T x T x 3 16x16x16 products
     (T = ceil(2N/16), three products per tile for the compensated split)."""
-    import re
-    vpw = int(re.search(r"_v(\d+)t", kernel).group(1))
+    vpw = MFMA_VPW[tuple(shape)]
     V = shape[0] * shape[1] * shape[2]
     per_block = 0
     for n in shape:
@@ -271,6 +275,16 @@ def precision_mix(pkg, steps=20):
     res["one_launch_fft_c64"] = dict(kernel=mp.info()[0], us=t * 1e6, GBs=2 * nbytes / t / 1e9,
                                      errors=errors(outs[0]))
     mp.close()
+    for tag, flag in (("one_launch_mfma_bf16x3", f.MFMA_BF16), ("one_launch_mfma_fp16x3", f.MFMA_FP16)):
+        mp = f.MultiPlan([(sh, b, -1) for sh, b in C4_ITEMS], f.C64, flags=flag)
+        t = timed(lambda s_: mp.execute([b.addr for b in ins[s_]], [b.addr for b in outs[s_]],
+                                        stream=stream))
+        h.synchronize()
+        fl = sum(mfma_flops(sh, b) for sh, b in C4_ITEMS)
+        res[tag] = dict(kernel=mp.info()[0], us=t * 1e6, GBs=2 * nbytes / t / 1e9,
+                        mfma_tflops=fl / t / 1e12, mfma_util=fl / t / 1e12 / PEAK_MFMA_TFLOPS,
+                        errors=errors(outs[0]))
+        mp.close()
     for tag, flag in (("fft_c64", 0), ("mfma_bf16x3", f.MFMA_BF16), ("mfma_fp16x3", f.MFMA_FP16)):
         per = {}
         for k, ((sh, b), x) in enumerate(zip(C4_ITEMS, xs)):
@@ -283,7 +297,7 @@ def precision_mix(pkg, steps=20):
             d["rel_l2"] = _rel_l2(y, ref)
             d["rel_max"] = _rel_max(y, ref)
             if flag:
-                fl = mfma_flops(p.kernel, sh, b)
+                fl = mfma_flops(sh, b)
                 d["mfma_tflops"] = fl / tk / 1e12
                 d["mfma_util"] = d["mfma_tflops"] / PEAK_MFMA_TFLOPS
             per[f"L{sh[0]}"] = d

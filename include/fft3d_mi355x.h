@@ -135,8 +135,9 @@ typedef struct fft3d_mx_item {
 } fft3d_mx_item;
 typedef struct fft3d_mx_multi fft3d_mx_multi;
 
-/* At most 16 items.  flags: FFT3D_MX_NORMALIZE (1/V on every item), FFT3D_MX_NO_TUNE.
- * $FFT3D_MX_MULTI_KERNEL=<name> forces a preset.  NULL on failure. */
+/* At most 16 items.  flags: FFT3D_MX_NORMALIZE (1/V on every item), FFT3D_MX_NO_TUNE, and
+ * for complex64 FFT3D_MX_MFMA_BF16 or FFT3D_MX_MFMA_FP16 (the matrix-core form in the same
+ * single launch).  $FFT3D_MX_MULTI_KERNEL=<name> forces a preset.  NULL on failure. */
 fft3d_mx_multi *fft3d_mx_multi_create(const fft3d_mx_item *items, int nitems, int precision,
                                       unsigned flags);
 int fft3d_mx_multi_execute(fft3d_mx_multi *m, const void *const *in, void *const *out,

@@ -155,20 +155,25 @@ struct MfmaCfg : C {
     static constexpr int SPLITV = SPLIT;
 };
 
+// One block of VPW volumes: stage, the three axis passes, store.  CONJ: conjugate on load
+// and on store (the backward transform through forward matrices, for the mixed launch).
+// Ends with a barrier, so a caller may reuse `sm` at once.
 template <class C, int SPLIT>
-__global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE, 8)))
-    k_mfma(const cx<float>* __restrict__ in, cx<float>* __restrict__ out, long nvol,
-           const Bank<float>* __restrict__ bk, float scl)
+__device__ __forceinline__ void mfma_block(long bid, long nvol, const cx<float>* __restrict__ in,
+                                           cx<float>* __restrict__ out,
+                                           const Bank<float>* __restrict__ bk, float scl,
+                                           bool conj, cx<float>* sm)
 {
-    static_assert(sizeof(typename C::T) == 4, "the matrix-core variant is complex64");
     using K = Fused<C>;
-    __shared__ cx<float> sm[C::LDS];
     const int tid = threadIdx.x;
-    const long bid = blockIdx.x;
     const int lane = tid & 63;
     const int nv = K::nvols(bid, nvol);
     K::stage(tid, bid, nvol, in, sm);
     __syncthreads();
+    if (conj) {
+        for (int e = tid; e < nv * C::V; e += C::NT) sm[K::slot(e)].i = -sm[K::slot(e)].i;
+        __syncthreads();
+    }
     {
         MfmaMatrix<C::N2, C::S, SPLIT> A;
         A.build(lane, bk);
@@ -209,7 +214,22 @@ __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::W
     }
     __syncthreads();
     cx<float>* __restrict__ dst = out + bid * C::VPW * (long)C::V;
-    for (int e = tid; e < nv * C::V; e += C::NT) dst[e] = sm[K::slot(e)];
+    const float ci = conj ? -1.f : 1.f;
+    for (int e = tid; e < nv * C::V; e += C::NT) {
+        const cx<float> y = sm[K::slot(e)];
+        dst[e] = cx<float>{y.r, ci * y.i};
+    }
+    __syncthreads();
+}
+
+template <class C, int SPLIT>
+__global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE, 8)))
+    k_mfma(const cx<float>* __restrict__ in, cx<float>* __restrict__ out, long nvol,
+           const Bank<float>* __restrict__ bk, float scl)
+{
+    static_assert(sizeof(typename C::T) == 4, "the matrix-core variant is complex64");
+    __shared__ cx<float> sm[C::LDS];
+    mfma_block<C, SPLIT>(blockIdx.x, nvol, in, out, bk, scl, false, sm);
 }
 
 }  // namespace fx

@@ -1,6 +1,8 @@
 // Instantiations of the mixed-batch persistent kernel (multi.hpp): both precisions, two
-// tile presets each, raced at create time.
+// tile presets each, raced at create time; and the complex64 matrix-core form
+// (multi_mfma.hpp), selected by the plan flags FFT3D_MX_MFMA_BF16 / FFT3D_MX_MFMA_FP16.
 #include "multi.hpp"
+#include "multi_mfma.hpp"
 
 namespace fx {
 
@@ -10,6 +12,8 @@ void register_multi(std::vector<MultiDesc>& r)
     r.push_back(multi_desc<float, 1>("multi_c64_t640"));
     r.push_back(multi_desc<double, 0>("multi_c128_t320"));
     r.push_back(multi_desc<double, 1>("multi_c128_t640"));
+    r.push_back(multi_mfma_desc<0>("multi_c64_mfma_t256_bf16x3"));
+    r.push_back(multi_mfma_desc<1>("multi_c64_mfma_t256_fp16x3"));
 }
 
 }  // namespace fx

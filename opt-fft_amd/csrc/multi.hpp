@@ -190,8 +190,9 @@ struct MultiDesc {
     size_t lds_bytes;
     const char* name;
     MultiLaunchFn launch;
-    MultiDevLaunchFn launch_dev;
+    MultiDevLaunchFn launch_dev;  // NULL: no descriptor form
     int (*resident_grid)();  // persistent grid of the descriptor form on the current device
+    int algo = 0;            // as KernelDesc::algo: 0 FFT, 1 / 2 matrix-core bf16 / fp16
 };
 
 void register_multi(std::vector<MultiDesc>& r);

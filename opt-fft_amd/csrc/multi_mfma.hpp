@@ -1,0 +1,59 @@
+// Mixed-geometry launch of the DFT-as-matrix variant (complex64, mfma.hpp): the items and
+// task list of multi.hpp, each task one block of mfma_block with the forward matrices
+// (backward items conjugate on load and store).
+#pragma once
+#include "mfma.hpp"
+#include "multi.hpp"
+
+namespace fx {
+
+// Tiles of the standalone matrix-core kernels (mfma.hip), one workgroup size.
+struct MultiMfmaPreset {
+    static constexpr int NT = 256;
+    using G0 = FusedCfg<6, 6, 6, float, -1, 16, NT, 1, 0, 0>;
+    using G1 = FusedCfg<8, 8, 8, float, -1, 8, NT, 1, 0, 0>;
+    using G2 = FusedCfg<12, 12, 12, float, -1, 2, NT, 1, 0, 0>;
+    using G3 = FusedCfg<17, 17, 17, float, -1, 1, NT, 0, 0, 0>;
+    using G4 = FusedCfg<8, 8, 16, float, -1, 4, NT, 1, 0, 0>;
+    static constexpr int LDS =
+        cmax(cmax(cmax(G0::LDS, G1::LDS), cmax(G2::LDS, G3::LDS)), G4::LDS);
+    static constexpr int WPE = waves_per_eu(LDS * (int)sizeof(cx<float>), NT, 0);
+    static constexpr int vpw(int code)
+    {
+        return code == 0 ? G0::VPW : code == 1 ? G1::VPW : code == 2 ? G2::VPW
+             : code == 3 ? G3::VPW : G4::VPW;
+    }
+};
+
+// One task per workgroup (grid = ntasks; the hardware dispatcher hands tasks, heaviest
+// first, to whichever CU frees up).  A persistent loop over tasks would keep the matrix
+// fragments of several geometries live across iterations and spill.
+template <int SPLIT>
+__global__ void __launch_bounds__(MultiMfmaPreset::NT)
+    __attribute__((amdgpu_waves_per_eu(MultiMfmaPreset::WPE, 8)))
+    k_multi_mfma(const MultiArgs<float> a, const int2* __restrict__ tasks, int ntasks,
+                 const Bank<float>* __restrict__ bk)
+{
+    using P = MultiMfmaPreset;
+    __shared__ cx<float> sm[P::LDS];
+    const long t = blockIdx.x;
+    if (t >= ntasks) return;
+    const int2 task = tasks[t];
+    const MultiItem<float>* ip = &a.item[task.x];
+    const cx<float>* in = ip->in;
+    cx<float>* out = ip->out;
+    const long nvol = ip->nvol;
+    const bool cj = ip->conj != 0;
+    const float scl = ip->scale;
+    switch (ip->code) {  // uniform over the workgroup
+    case 0: mfma_block<P::G0, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
+    case 1: mfma_block<P::G1, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
+    case 2: mfma_block<P::G2, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
+    case 3: mfma_block<P::G3, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
+    default: mfma_block<P::G4, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
+    }
+}
+
+}  // namespace fx
+
+#include <fx_multi_mfma_launch.hpp>

@@ -168,6 +168,10 @@ extern "C" fft3d_mx_multi* fft3d_mx_multi_create(const fft3d_mx_item* items, int
         fail(FFT3D_MX_EINVAL, "fft3d_mx_multi_create: unknown precision %d", precision);
         return nullptr;
     }
+    if ((flags & FFT3D_MX_MFMA_BF16) && (flags & FFT3D_MX_MFMA_FP16)) {
+        fail(FFT3D_MX_EINVAL, "fft3d_mx_multi_create: MFMA_BF16 and MFMA_FP16 are exclusive");
+        return nullptr;
+    }
     fft3d_mx_multi* m = new (std::nothrow) fft3d_mx_multi();
     if (!m) {
         fail(FFT3D_MX_ENOMEM, "fft3d_mx_multi_create: out of host memory");
@@ -205,13 +209,17 @@ extern "C" fft3d_mx_multi* fft3d_mx_multi_create(const fft3d_mx_item* items, int
         delete m;
         return nullptr;
     }
+    const int algo = (flags & FFT3D_MX_MFMA_BF16) ? 1 : (flags & FFT3D_MX_MFMA_FP16) ? 2 : 0;
     std::vector<const MultiDesc*> cands;
     const char* forced = getenv("FFT3D_MX_MULTI_KERNEL");
     for (const MultiDesc& d : multi_registry())
-        if (d.prec == precision && (!forced || !*forced || !strcmp(forced, d.name)))
+        if (d.prec == precision && d.algo == algo &&
+            (!forced || !*forced || !strcmp(forced, d.name)))
             cands.push_back(&d);
     if (cands.empty()) {
-        fail(FFT3D_MX_EUNSUPPORTED, "fft3d_mx_multi_create: no mixed kernel named %s", forced);
+        fail(FFT3D_MX_EUNSUPPORTED, "fft3d_mx_multi_create: no %smixed kernel%s%s for %s",
+             algo ? "matrix-core " : "", forced && *forced ? " named " : "",
+             forced && *forced ? forced : "", precision == kC128 ? "complex128" : "complex64");
         (void)hipFree(m->bank);
         (void)hipFree(m->work);
         delete m;

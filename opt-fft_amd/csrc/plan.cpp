@@ -163,10 +163,13 @@ static const KernelDesc* race(const std::vector<const KernelDesc*>& cands, int n
     if (hipStreamCreate(&s) == hipSuccess) {
         if (hipEventCreate(&e0) == hipSuccess) {
             if (hipEventCreate(&e1) == hipSuccess) {
+                // A later candidate must beat the incumbent by more than 2% (about the
+                // run-to-run spread of one trial), so near-ties resolve the same way each
+                // time, to the earlier-registered variant.
                 for (const KernelDesc* k : cands) {
                     const double t = trial(*k, (const char*)din, (char*)dout, wbytes, nwin,
                                            rb, bank, work, s, e0, e1);
-                    if (t > 0 && t < wt) {
+                    if (t > 0 && (wt > 1e29 || t < 0.98 * wt)) {
                         wt = t;
                         win = k;
                     }

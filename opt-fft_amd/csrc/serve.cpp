@@ -471,7 +471,7 @@ extern "C" fft3d_srv* fft3d_srv_create(const fft3d_srv_config* cfg)
         ok = hipMalloc(&s->bank[prec], img.size()) == hipSuccess &&
              hipMemcpy(s->bank[prec], img.data(), img.size(), hipMemcpyHostToDevice) == hipSuccess;
         for (const MultiDesc& d : multi_registry())
-            if (d.prec == prec && !s->mk[prec]) s->mk[prec] = &d;
+            if (d.prec == prec && d.algo == 0 && d.launch_dev && !s->mk[prec]) s->mk[prec] = &d;
         ok = ok && s->mk[prec] && make_slots(s, prec, cfg->streams);
     }
     // Plans for every geometry, both precisions and directions (the plain path uses them

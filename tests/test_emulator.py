@@ -115,6 +115,8 @@ def test_mixed_batch_kernels_on_the_emulator(emu):
     assert emu.emu_multi_count() >= 4
     for i in range(emu.emu_multi_count()):
         name = emu.emu_multi_name(i)
+        if b"mfma" in name:
+            continue  # wave-level matrix-core kernel: checked on the GPU only
         prec = 0 if b"c128" in name else 1
         dt = np.complex128 if prec == 0 else np.complex64
         xs = [oracle.make_input(s, max(b, 1), seed=k, dtype=dt)[:b] for k, (s, b, d) in enumerate(MIX)]
