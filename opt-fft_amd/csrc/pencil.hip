@@ -18,12 +18,9 @@ void register_pencil(std::vector<KernelDesc>& r)
 {
     // L = 36: slab 36x36 = 20.7 KB complex128.
     FX_P_BOTH(36, 36, 36, double, "c128", 3, 128, 1, 128, 0, 2);
-    FX_P_BOTH(36, 36, 36, double, "c128", 2, 128, 1, 128, 0, 3);
     FX_P_BOTH(36, 36, 36, double, "c128", 7, 256, 1, 128, 0, 1);
     FX_P_BOTH(36, 36, 36, double, "c128", 3, 128, 1, 128, 0, 0);
     FX_P_BOTH(36, 36, 36, double, "c128", 7, 256, 1, 128, 0, 0);
-    FX_P_BOTH(36, 36, 36, double, "c128", 3, 128, 1, 128, 32, 2);
-    FX_P_BOTH(36, 36, 36, double, "c128", 3, 128, 1, 128, 16, 2);
     FX_P_BOTH(36, 36, 36, float, "c64", 7, 256, 1, 128, 0, 1);
     FX_P_BOTH(36, 36, 36, float, "c64", 3, 128, 1, 128, 0, 4);
     // L = 24: slab 24x24 = 9.2 KB complex128.
@@ -33,10 +30,8 @@ void register_pencil(std::vector<KernelDesc>& r)
     FX_P_BOTH(24, 24, 24, float, "c64", 16, 384, 1, 128, 0, 2);
     FX_P_BOTH(24, 24, 24, float, "c64", 16, 384, 1, 128, 0, 0);
     // 24 x 24 x 48: slab 24x48 = 18.4 KB complex128; z lines 48-point, y and x 24-point.
-    FX_P_BOTH(24, 24, 48, double, "c128", 4, 192, 1, 128, 0, 2);
     FX_P_BOTH(24, 24, 48, double, "c128", 4, 128, 1, 128, 0, 0);
     FX_P_BOTH(24, 24, 48, double, "c128", 8, 256, 1, 128, 0, 0);
-    FX_P_BOTH(24, 24, 48, double, "c128", 4, 192, 1, 128, 32, 2);
     FX_P_BOTH(24, 24, 48, float, "c64", 8, 384, 1, 128, 0, 2);
     FX_P_BOTH(24, 24, 48, float, "c64", 8, 256, 1, 128, 0, 0);
 }
