@@ -367,6 +367,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--workload", default=None,
+                    help="time only this secondary workload (e.g. L8_c128_fwd_b4096) and print "
+                         "its measurement; for profiling runs, no JSON line")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; batches of independent volumes "
@@ -384,6 +387,13 @@ def main():
         sys.exit(1)
     pkg.hip.check(pkg.hip.lib().hipSetDevice(0), "hipSetDevice")
 
+    if a.workload:
+        wl = [w for w in SECONDARY if w["name"] == a.workload]
+        if not wl:
+            print(f"bench.py: unknown workload {a.workload}", file=sys.stderr)
+            sys.exit(2)
+        print(f"# {a.workload}: {measure(pkg, wl[0], a.steps, a.warmup)}", flush=True)
+        return
     head = measure(pkg, HEADLINE, a.steps, a.warmup)
     print(f"# headline {HEADLINE['name']}: {head}", file=sys.stderr, flush=True)
     secondary = {}

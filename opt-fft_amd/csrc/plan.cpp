@@ -98,10 +98,12 @@ static std::string wisdom_key(int n0, int n1, int n2, int prec, int dir, long ba
     return buf;
 }
 
-// Time one candidate: 2 warm launches, then the best of 3 groups of 5.  Successive launches
-// use successive windows of the race buffers, so with enough windows no launch finds its
-// data in the 256 MB Infinity Cache and the race ranks kernels as they run from HBM.
-// Seconds per launch, or -1 if it failed to launch.
+// Time one candidate: 2 warm launches, one group of 5 to size the groups, then the best of
+// 3 groups of n launches, n chosen so a group lasts about 300 us (5..64): event timing
+// jitter of a few microseconds would otherwise decide between kernels of ~10 us.
+// Successive launches use successive windows of the race buffers, so with enough windows
+// no launch finds its data in the 256 MB Infinity Cache and the race ranks kernels as they
+// run from HBM.  Seconds per launch, or -1 if it failed to launch.
 static double trial(const KernelDesc& k, const char* din, char* dout, size_t wbytes, int nwin,
                     long nvol, const void* bank, unsigned* work, hipStream_t s, hipEvent_t e0,
                     hipEvent_t e1)
@@ -113,16 +115,24 @@ static double trial(const KernelDesc& k, const char* din, char* dout, size_t wby
     };
     for (int i = 0; i < 2; ++i)
         if (launch() != hipSuccess) return -1.0;
-    double best = 1e30;
-    for (int g = 0; g < 3; ++g) {
+    auto group = [&](int n) -> double {
         if (hipEventRecord(e0, s) != hipSuccess) return -1.0;
-        for (int i = 0; i < 5; ++i)
+        for (int i = 0; i < n; ++i)
             if (launch() != hipSuccess) return -1.0;
         if (hipEventRecord(e1, s) != hipSuccess) return -1.0;
         if (hipEventSynchronize(e1) != hipSuccess) return -1.0;
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
-        const double per = (double)ms * 1e-3 / 5.0;
+        return (double)ms * 1e-3 / n;
+    };
+    const double first = group(5);
+    if (first <= 0) return -1.0;
+    int n = (int)(300e-6 / first);
+    n = n < 5 ? 5 : (n > 64 ? 64 : n);
+    double best = first;
+    for (int g = 0; g < 3; ++g) {
+        const double per = group(n);
+        if (per <= 0) return -1.0;
         if (per < best) best = per;
     }
     return best;
