@@ -129,7 +129,7 @@ static double trial(const KernelDesc& k, const char* din, char* dout, size_t wby
     if (first <= 0) return -1.0;
     int n = (int)(300e-6 / first);
     n = n < 5 ? 5 : (n > 64 ? 64 : n);
-    double best = first;
+    double best = 1e30;
     for (int g = 0; g < 3; ++g) {
         const double per = group(n);
         if (per <= 0) return -1.0;
