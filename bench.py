@@ -155,7 +155,8 @@ def _ref_exe(kind):
 def cpu_baseline(seconds=10.0):
     """The reference's own CPU kernels on this host, on a bounded sample of the headline
     workload (17^3 complex128): all cores through its OpenMP kernel, plus one pinned core
-    through its best-exemplar library.  Falls back to the oracle's numpy.fft ("port")."""
+    through its best-exemplar library; the same two for 8^3 and 36^3 on 3 s samples
+    ("other_geometries").  Falls back to the oracle's numpy.fft ("port")."""
     import subprocess
     mt, st = _ref_exe("mt"), _ref_exe("st")
     if mt and st:
@@ -167,6 +168,20 @@ def cpu_baseline(seconds=10.0):
         r1 = subprocess.run(["taskset", "-c", "0", st, "--L", "17", "--batch", "256",
                              "--seconds", str(seconds / 2)], capture_output=True, text=True,
                             timeout=300)
+        # The other configurations' geometries, shorter samples: all cores and one core.
+        others = {}
+        for L, b_mt, b_st in ((8, 4096, 2048), (36, 128, 32)):
+            rm = subprocess.run([mt, "--L", str(L), "--batch", str(b_mt), "--seconds", "3"],
+                                capture_output=True, text=True, env=env, timeout=300)
+            rs = subprocess.run(["taskset", "-c", "0", st, "--L", str(L), "--batch", str(b_st),
+                                 "--seconds", "3"], capture_output=True, text=True, timeout=300)
+            if rm.returncode == 0 and rs.returncode == 0:
+                a_, b_ = (json.loads(x.stdout.strip().splitlines()[-1]) for x in (rm, rs))
+                others[f"L{L}"] = dict(
+                    all_cores=dict(value=a_["transforms_per_s"], cores=a_["threads"],
+                                   kernel=a_["kernel"], batch=b_mt, seconds=a_["seconds"]),
+                    single_core=dict(value=b_["transforms_per_s"], cores=1,
+                                     kernel=b_["kernel"], batch=b_st, seconds=b_["seconds"]))
         if r.returncode == 0 and r1.returncode == 0:
             m, s1 = json.loads(r.stdout.strip().splitlines()[-1]), json.loads(r1.stdout.strip().splitlines()[-1])
             model = ""
@@ -185,7 +200,8 @@ def cpu_baseline(seconds=10.0):
                                          sample=f"reference best-exemplar library "
                                                 f"(bench/geom/best, picked {s1['kernel']}) "
                                                 f"on 17^3 x 256, taskset -c 0, "
-                                                f"{s1['seconds']:.1f} s"))
+                                                f"{s1['seconds']:.1f} s"),
+                        other_geometries=others)
     import oracle
     x = make_input((17, 17, 17), 64, seed=7, dist="uniform")
     n, t0 = 0, time.perf_counter()
