@@ -23,7 +23,7 @@ within 2 ms, for continuous batching and for the plain one-launch-per-request pa
 the throughput ratio of the two at that cap.
 
 Prints ONE JSON line.  `python bench.py --gpus N` with N > 1 exits: this build runs on one
-GPU (batches shard by volume with no communication).
+GPU (batches of independent volumes shard by volume with no communication).
 """
 import argparse
 import json
@@ -389,7 +389,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; batches of independent volumes "
-              "shard across GPUs with no communication (see DESIGN.md).", file=sys.stderr)
+              "shard across GPUs with no communication.", file=sys.stderr)
         sys.exit(2)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     # The CPU baseline spawns the reference binaries, so it runs before HIP is initialised.

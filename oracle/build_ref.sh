@@ -1,7 +1,7 @@
 #!/bin/bash
 # Reference CPU baseline, compiled from the reference's own C sources where they lie under
 # $REF_DIR (never copied into this repository).  Outputs only under oracle/_ref/, which is
-# git-ignored and travels to the GPU box with the tree.
+# git-ignored and travels with the tree to the machine that runs the benchmark.
 #
 #   _ref/<isa>/ref_bench_st  single thread: the reference's best-exemplar library
 #                            bench/geom/best/fft3d_best.c (race-at-create over its kernels)

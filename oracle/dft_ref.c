@@ -110,7 +110,7 @@ int oracle_dft3d_c128(const double *in, double *out, long batch,
 }
 
 /* Same transform for complex64 input; the result is returned in complex128 so that a
- * single-precision kernel can be judged against the exact answer, not a rounded one. */
+ * single-precision kernel can be checked against the exact answer, not a rounded one. */
 int oracle_dft3d_c64(const float *in, double *out, long batch,
                      int n0, int n1, int n2, int sign, int normalize)
 {

@@ -1,7 +1,7 @@
 """GPU parity: every compiled kernel variant against the oracle, through the C ABI.
 
 Tolerances (relative L2 over the whole batch, bench/geom/check.py:29-30):
-  complex128  1e-13   (observed ~5e-16)
+  complex128  1e-13   (the path's bar; observed ~5e-16)
   complex64   2e-6    (single precision: ~1e-7 observed; oracle is exact long double)
 """
 import numpy as np
