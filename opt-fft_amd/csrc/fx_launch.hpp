@@ -1,6 +1,8 @@
 // Device launchers and registry descriptors for the kernel templates of kernels.hpp.
 // Included at the end of kernels.hpp through the include path.
 #pragma once
+#include <mutex>
+#include <unordered_map>
 
 namespace fx {
 
@@ -20,17 +22,22 @@ inline int cu_count()
     return cache[dev];
 }
 
-// Workgroups of kernel `f` (nt threads) resident per CU, cached per kernel instantiation.
+// Workgroups of kernel `f` (nt threads) resident per CU, cached per kernel (by address:
+// kernels of one signature share a function pointer type, not an occupancy).
 template <typename F>
 int resident_per_cu(F f, int nt)
 {
-    static int occ = 0;
-    if (!occ) {
-        int o = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, f, nt, 0) != hipSuccess || o < 1) o = 1;
-        occ = o;
-    }
-    return occ;
+    static std::mutex mu;
+    static std::unordered_map<const void*, int> cache;
+    const void* key = reinterpret_cast<const void*>(f);
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int o = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, key, nt, 0) != hipSuccess || o < 1)
+        o = 1;
+    cache.emplace(key, o);
+    return o;
 }
 
 // Grid of a persistent kernel: at most `cap` workgroups per CU, never more than nblocks.

@@ -39,13 +39,14 @@ def test_every_kernel_variant_matches_oracle(gpu):
         # batch sizes: one volume, and a batch that leaves the last workgroup partial
         for batch in (1, 5):
             x = oracle.make_input(shape, batch, seed=101 + batch, dist="uniform", dtype=dt)
+            print(f"{name} B={batch} ...", end=" ", flush=True)
             plan = f.Plan(shape, batch, direction, prec, kernel=name)
             assert plan.kernel == name
             y = plan.run(x)
             plan.close()
             ref = oracle.fftn_numpy(x, sign=direction)
             err = oracle.rel_l2(y, ref)
-            print(f"{name} B={batch} rel_l2={err:.2e}", flush=True)
+            print(f"rel_l2={err:.2e}", flush=True)
             assert np.all(np.isfinite(y)), name
             assert err <= _tol(name, prec), f"{name} B={batch}: rel_l2 {err:.3e}"
             worst[name] = max(worst.get(name, 0.0), err)
