@@ -4,12 +4,18 @@
 namespace fx {
 
 template <typename T, int PRESET>
+long multi_launch_grid(long ntasks)
+{
+    return persistent_grid(k_multi<T, PRESET>, MultiCfg<T, PRESET>::NT, 8, ntasks);
+}
+
+template <typename T, int PRESET>
 hipError_t launch_multi(const void* args, const int2* tasks, int ntasks, const void* bank,
                         unsigned* work, hipStream_t stream)
 {
     using M = MultiCfg<T, PRESET>;
     if (ntasks <= 0) return hipSuccess;
-    const long grid = persistent_grid(k_multi<T, PRESET>, M::NT, 8, ntasks);
+    const long grid = multi_launch_grid<T, PRESET>(ntasks);
     hipLaunchKernelGGL((k_multi<T, PRESET>), dim3((unsigned)grid), dim3(M::NT), 0, stream,
                        *(const MultiArgs<T>*)args, tasks, ntasks, (const Bank<T>*)bank,
                        claim_counter(work, grid, ntasks));
@@ -39,7 +45,8 @@ MultiDesc multi_desc(const char* name)
     return MultiDesc{prec_of<T>(), PRESET, M::NT,
                      {M::vpw(0), M::vpw(1), M::vpw(2), M::vpw(3), M::vpw(4)},
                      sizeof(cx<T>) * (size_t)M::LDS, name, &launch_multi<T, PRESET>,
-                     &launch_multi_dev<T, PRESET>, &multi_dev_grid<T, PRESET>};
+                     &launch_multi_dev<T, PRESET>, &multi_dev_grid<T, PRESET>, 0,
+                     &multi_launch_grid<T, PRESET>};
 }
 
 }  // namespace fx

@@ -14,13 +14,16 @@ hipError_t launch_multi_mfma(const void* args, const int2* tasks, int ntasks, co
     return hipGetLastError();
 }
 
+// One workgroup per task.
+inline long multi_mfma_grid(long ntasks) { return ntasks; }
+
 template <int SPLIT>
 MultiDesc multi_mfma_desc(const char* name)
 {
     using P = MultiMfmaPreset;
     return MultiDesc{kC64, 2 + SPLIT, P::NT, {P::vpw(0), P::vpw(1), P::vpw(2), P::vpw(3), P::vpw(4)},
                      sizeof(cx<float>) * (size_t)P::LDS, name, &launch_multi_mfma<SPLIT>,
-                     nullptr, nullptr, 1 + SPLIT};
+                     nullptr, nullptr, 1 + SPLIT, &multi_mfma_grid};
 }
 
 }  // namespace fx

@@ -193,6 +193,7 @@ struct MultiDesc {
     MultiDevLaunchFn launch_dev;  // NULL: no descriptor form
     int (*resident_grid)();  // persistent grid of the descriptor form on the current device
     int algo = 0;            // as KernelDesc::algo: 0 FFT, 1 / 2 matrix-core bf16 / fp16
+    long (*launch_grid)(long ntasks) = nullptr;  // workgroups `launch` starts for ntasks
 };
 
 void register_multi(std::vector<MultiDesc>& r);

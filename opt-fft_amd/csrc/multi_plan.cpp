@@ -119,8 +119,10 @@ static const MultiDesc* multi_race(fft3d_mx_multi* m, const std::vector<const Mu
         hipEvent_t e0, e1;
         hipStream_t s;
         double best = 1e30;
-        if (hipStreamCreate(&s) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
-            hipEventCreate(&e1) == hipSuccess) {
+        const bool have_s = hipStreamCreate(&s) == hipSuccess;
+        const bool have_e0 = have_s && hipEventCreate(&e0) == hipSuccess;
+        const bool have_e1 = have_e0 && hipEventCreate(&e1) == hipSuccess;
+        if (have_e1) {
             for (const MultiDesc* k : cands) {
                 std::vector<int2> t;
                 multi_tasks(*k, m->code, m->batch, m->nitems, elem, t);
@@ -145,10 +147,10 @@ static const MultiDesc* multi_race(fft3d_mx_multi* m, const std::vector<const Mu
                     win = k;
                 }
             }
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            (void)hipStreamDestroy(s);
         }
+        if (have_e1) (void)hipEventDestroy(e1);
+        if (have_e0) (void)hipEventDestroy(e0);
+        if (have_s) (void)hipStreamDestroy(s);
         std::memcpy(m->batch, saved, sizeof saved);
     }
     for (void* b : bufs) (void)hipFree(b);
@@ -263,7 +265,13 @@ extern "C" int fft3d_mx_multi_get_info(const fft3d_mx_multi* m, char* kernel, si
     if (!m) return fail(FFT3D_MX_EINVAL, "fft3d_mx_multi_get_info: NULL plan");
     if (kernel && cap) snprintf(kernel, cap, "%s", m->k->name);
     if (ntasks) *ntasks = m->ntasks;
-    if (grid) *grid = 0;
+    if (grid) {
+        int cur = m->device;
+        (void)hipGetDevice(&cur);
+        if (cur != m->device) (void)hipSetDevice(m->device);
+        *grid = m->k->launch_grid && m->ntasks > 0 ? (int)m->k->launch_grid(m->ntasks) : 0;
+        if (cur != m->device) (void)hipSetDevice(cur);
+    }
     return FFT3D_MX_OK;
 }
 

@@ -230,14 +230,19 @@ static bool launch_plan(fft3d_srv* s, Req* q, int stream, bool sync)
     return true;
 }
 
-// Copy the scheduler's counters to the snapshot fft3d_srv_get_stats reads.
-static void publish(fft3d_srv* s)
+// Copy the scheduler's counters to the snapshot fft3d_srv_get_stats reads (caller holds mu).
+static void publish_locked(fft3d_srv* s)
 {
-    std::lock_guard<std::mutex> lk(s->mu);
     s->st_pub = s->st;
     s->st_pub.submitted = s->submitted;
     s->st_pub.mean_items_per_mixed_launch =
         s->st.mixed_launches ? (double)s->items_in_mixed / (double)s->st.mixed_launches : 0.0;
+}
+
+static void publish(fft3d_srv* s)
+{
+    std::lock_guard<std::mutex> lk(s->mu);
+    publish_locked(s);
 }
 
 // t < 0 marks a request that could not be launched.  The waiter is released by the caller
@@ -250,15 +255,24 @@ static void finish(fft3d_srv* s, Req* q, double t)
     s->st.completed++;
 }
 
-static void release(Req* q) { q->done.store(1, std::memory_order_release); }
+// Publish the statistics that count these completions and mark them done, both under mu, so
+// a waiter that tested its flag under mu before the mark is already waiting on cv_done when
+// the notification comes.
+static void release(fft3d_srv* s, const std::vector<Req*>& reqs)
+{
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        publish_locked(s);
+        for (Req* q : reqs) q->done.store(1, std::memory_order_release);
+    }
+    s->cv_done.notify_all();
+}
 
 // Requests that could not be launched complete at once with a failure mark.
 static void fail_reqs(fft3d_srv* s, const std::vector<Req*>& reqs)
 {
     for (Req* q : reqs) finish(s, q, -1.0);
-    publish(s);
-    for (Req* q : reqs) release(q);
-    s->cv_done.notify_all();
+    release(s, reqs);
 }
 
 static void retire(fft3d_srv* s)
@@ -282,9 +296,7 @@ static void retire(fft3d_srv* s)
         s->inflight.pop_back();
     }
     if (done.empty()) return;
-    publish(s);
-    for (Req* q : done) release(q);
-    s->cv_done.notify_all();
+    release(s, done);
 }
 
 static int free_stream(fft3d_srv* s)
@@ -537,9 +549,9 @@ extern "C" int fft3d_srv_wait(fft3d_srv* s, long id, double* latency)
         if (id < 0 || id >= (long)s->all.size()) return fail(FFT3D_MX_EINVAL, "fft3d_srv_wait: bad id");
         q = s->all[id].get();
     }
-    while (!q->done.load(std::memory_order_acquire)) {
+    {
         std::unique_lock<std::mutex> lk(s->mu);
-        s->cv_done.wait_for(lk, std::chrono::microseconds(200));
+        s->cv_done.wait(lk, [q] { return q->done.load(std::memory_order_acquire) != 0; });
     }
     if (q->t_done < 0) return fail(FFT3D_MX_EHIP, "fft3d_srv_wait: request %ld failed to launch", id);
     if (latency) *latency = q->t_done - q->t_arrive;

@@ -333,7 +333,7 @@ class MultiPlan:
         if load().fft3d_mx_multi_get_info(self.handle, name, 96, ctypes.byref(nt),
                                           ctypes.byref(gr)) != 0:
             raise ValueError(last_error())
-        return name.value.decode(), nt.value
+        return name.value.decode(), nt.value, gr.value
 
     def execute(self, ins, outs, stream=None):
         n = len(self.items)

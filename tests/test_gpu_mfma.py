@@ -62,15 +62,18 @@ def test_mfma_flags_rejected_where_unsupported(gpu):
 
 @pytest.mark.parametrize("split", ["bf16", "fp16"])
 def test_mfma_in_one_mixed_launch(gpu, split):
-    """Uneven batches of every fused geometry, both directions, one persistent matrix-core
-    launch; each item against the oracle at the variant's bar, and equal to the same item
-    through the single-geometry matrix-core plan to within rounding of the same products."""
+    """Uneven batches of every fused geometry, both directions, one matrix-core launch (one
+    workgroup per task); each item against the oracle at the variant's bar, and equal to the
+    same item through the single-geometry matrix-core plan to within rounding of the same
+    products."""
     f = gpu.fft3d
     flag = f.MFMA_BF16 if split == "bf16" else f.MFMA_FP16
     items = [((6, 6, 6), 37, -1), ((8, 8, 8), 19, 1), ((12, 12, 12), 5, -1),
              ((17, 17, 17), 3, 1), ((8, 8, 16), 9, -1), ((6, 6, 6), 1, 1)]
     mp = f.MultiPlan(items, f.C64, flags=flag)
-    assert split in mp.info()[0]
+    name, ntasks, grid = mp.info()
+    assert split in name
+    assert grid == ntasks > len(items), (grid, ntasks)
     xs = [oracle.make_input(sh, b, seed=31 + k, dtype=np.complex64)
           for k, (sh, b, d) in enumerate(items)]
     ys = mp.run(xs)

@@ -18,8 +18,10 @@ def test_mixed_batch_matches_oracle(gpu, prec):
     tol = 1e-13 if prec == 0 else 2e-6
     xs = [oracle.make_input(s, b, seed=11 + k, dtype=dt) for k, (s, b, d) in enumerate(MIX)]
     m = f.MultiPlan(MIX, precision=prec)
-    name, ntasks = m.info()
+    name, ntasks, grid = m.info()
     assert ntasks > len(MIX)
+    # persistent: at most 8 workgroups per CU (MI355X: 256 CUs), never more than the tasks
+    assert 0 < grid <= min(ntasks, 8 * 256), (grid, ntasks)
     ys = m.run(xs)
     for x, y, (s, b, d) in zip(xs, ys, MIX):
         err = oracle.rel_l2(y, oracle.fftn_numpy(x, sign=d))
