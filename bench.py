@@ -38,12 +38,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch of the headline kernel (pipe_17x17x17_c128_fwd_v1t320p1_fold, the
+# HBM bytes per launch of the headline kernel (pipe_17x17x17_c128_fwd_v1t320p1_fold_st, the
 # race's pick) from rocprofv3 FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on the
-# runtime's copy of a known size: 39,441.3 KiB x 2 + 78,654.8 KiB
+# runtime's copy of a known size: 39,438.8 KiB x 2 + 78,667.9 KiB
 # (profiles/k_fpipe_17x17x17_c128_fwd_b1024.md).
-TRAFFIC = int((39441.3 * 2 + 78654.8) * 1024)
-TRAFFIC_KERNEL = "pipe_17x17x17_c128_fwd_v1t320p1_fold"
+TRAFFIC = int((39438.8 * 2 + 78667.9) * 1024)
+TRAFFIC_KERNEL = "pipe_17x17x17_c128_fwd_v1t320p1_fold_st"
 ROTATE_BYTES = 2 << 30
 
 HEADLINE = dict(name="L17_c128_fwd_b1024", shape=(17, 17, 17), batch=1024, dist="uniform",

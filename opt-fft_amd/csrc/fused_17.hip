@@ -16,16 +16,24 @@ namespace fx {
 #define FX_17P_BOTH(T, TAG, VPW, NT, ALG, ATAG, P)                                                \
     FX_17P(T, -1, VPW, NT, ALG, P, "pipe_17x17x17_" TAG "_fwd_v" #VPW "t" #NT "p" #P "_" ATAG); \
     FX_17P(T, +1, VPW, NT, ALG, P, "pipe_17x17x17_" TAG "_bwd_v" #VPW "t" #NT "p" #P "_" ATAG)
+// Staggered HBM traffic (PipeCfg ST = 1): suffix "_st".
+#define FX_17S(T, S, VPW, NT, ALG, P, NAME) \
+    r.push_back(fpipe_desc<FusedCfg<17, 17, 17, T, S, VPW, NT, 0, 0, ALG>, P, 1>(NAME))
+#define FX_17S_BOTH(T, TAG, VPW, NT, ALG, ATAG, P)                                                          \
+    FX_17S(T, -1, VPW, NT, ALG, P, "pipe_17x17x17_" TAG "_fwd_v" #VPW "t" #NT "p" #P "_" ATAG "_st"); \
+    FX_17S(T, +1, VPW, NT, ALG, P, "pipe_17x17x17_" TAG "_bwd_v" #VPW "t" #NT "p" #P "_" ATAG "_st")
 
 void register_fused_17(std::vector<KernelDesc>& r)
 {
     // Fastest from HBM at batch 1024 first (the race keeps the earlier variant on near-ties).
+    FX_17S_BOTH(double, "c128", 1, 320, 1, "fold", 1);
     FX_17P_BOTH(double, "c128", 1, 320, 1, "fold", 1);
     FX_17P_BOTH(double, "c128", 2, 640, 1, "fold", 1);
     FX_17P_BOTH(double, "c128", 1, 320, 1, "fold", 0);
     FX_17_BOTH(double, "c128", 1, 320, 1, "fold");
     FX_17_BOTH(double, "c128", 1, 320, 0, "rader");
     FX_17_BOTH(double, "c128", 1, 192, 1, "fold");
+    FX_17S_BOTH(float, "c64", 1, 320, 1, "fold", 2);
     FX_17P_BOTH(float, "c64", 1, 320, 1, "fold", 4);
     FX_17P_BOTH(float, "c64", 2, 640, 1, "fold", 2);
     FX_17_BOTH(float, "c64", 2, 320, 1, "fold");

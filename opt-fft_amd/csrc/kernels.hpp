@@ -222,6 +222,8 @@ struct FusedPipe {
 
     // Issue the loads of block b's x-lines.  Lanes past the last line re-read a valid
     // element (their values are never used).
+    // Points J0 .. J1-1 of each line (all of them by default).
+    template <int J0 = 0, int J1 = C::N0>
     static FX_HD void load_x(int tid, long b, long nvol, const cx<T>* __restrict__ in, Regs& R)
     {
         const int nl = nvols(b, nvol) * C::PLANE;
@@ -234,7 +236,7 @@ struct FusedPipe {
             const int q = l - v * C::PLANE;
             const cx<T>* __restrict__ p = src + (long)v * C::V + q;
 #pragma unroll
-            for (int j = 0; j < C::N0; ++j) R[i][j] = p[j * C::PLANE];
+            for (int j = J0; j < J1; ++j) R[i][j] = p[j * C::PLANE];
         }
     }
     // x DFT in registers, results to their LDS slots.
@@ -292,10 +294,17 @@ struct FusedPipe {
 
 // A pipelined kernel launched with at most P workgroups per CU (P > 0) may spend the
 // registers of P resident workgroups rather than of as many as the LDS footprint allows.
-template <class C, int P>
+// ST = 1 staggers the HBM traffic: the next block's x-lines are issued in two halves (the
+// first ceil(N0/2) points after the x pass, the rest after the z pass) and odd workgroups
+// start ~1.6 us late, so the CUs' read and write phases spread out instead of every CU
+// reading at once.  17^3 complex128 at batch 1024: 38.1 us against 40.2 us; kernels of
+// ~15 us (8^3, 6^3, 8x8x16) lose more to the late start than they gain.
+template <class C, int P, int ST_ = 0>
 struct PipeCfg : C {
     static constexpr int WPE =
         P > 0 ? waves_per_eu(C::LDS * (int)sizeof(cx<typename C::T>), C::NT, P) : C::WPE;
+    static constexpr int ST = ST_;
+    static constexpr int SPLIT = ST_ ? (C::N0 + 1) / 2 : C::N0;
 };
 
 // Next block of a persistent workgroup.  Each workgroup starts at block blockIdx.x and then
@@ -324,6 +333,7 @@ __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::W
     const int tid = threadIdx.x;
     long b = blockIdx.x;
     if (b >= nblocks) return;
+    if (C::ST && (blockIdx.x & 1)) __builtin_amdgcn_s_sleep(60);  // 60 x 64 cycles
     K::load_x(tid, b, nvol, in, R);
     for (;;) {
         long claim = 0;
@@ -332,8 +342,10 @@ __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::W
         if (tid == 0) s_next = claim;
         __syncthreads();
         const long nb = s_next;
-        if (nb < nblocks) K::load_x(tid, nb, nvol, in, R);
+        if (nb < nblocks) K::template load_x<0, C::SPLIT>(tid, nb, nvol, in, R);
         K::zpass(tid, b, nvol, sm, bk);
+        if (C::SPLIT < C::N0 && nb < nblocks)
+            K::template load_x<C::SPLIT, C::N0>(tid, nb, nvol, in, R);
         __syncthreads();
         K::ypass(tid, b, nvol, sm, out, bk, scl);
         if (nb >= nblocks) break;
