@@ -357,12 +357,15 @@ __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::W
 // ------------------------------------------------------------------------------ pencil
 // PIPE = 0: pass A is k_slab (one slab group per workgroup); PIPE = P > 0: pass A is the
 // persistent pipelined k_spipe with at most P resident workgroups per CU.
+// ST = 1: k_spipe staggers its HBM traffic as PipeCfg ST does (y-lines of the next group
+// issued in two halves, odd workgroups starting ~1.6 us late).
 template <int N0_, int N1_, int N2_, typename T_, int S_, int SPW_, int NT_, int PZ_,
-          int NTB_, int CHUNK_, int PIPE_ = 0>
+          int NTB_, int CHUNK_, int PIPE_ = 0, int ST_ = 0>
 struct PencilCfg {
     using T = T_;
     static constexpr int N0 = N0_, N1 = N1_, N2 = N2_, S = S_, SPW = SPW_, NT = NT_;
-    static constexpr int PZ = PZ_, NTB = NTB_, CHUNK = CHUNK_, PIPE = PIPE_;
+    static constexpr int PZ = PZ_, NTB = NTB_, CHUNK = CHUNK_, PIPE = PIPE_, ST = ST_;
+    static constexpr int SPLIT = ST_ ? (N1_ + 1) / 2 : N1_;  // y points in the first half
     static constexpr int V = N0 * N1 * N2;
     static constexpr int PLANE = N1 * N2;
     static constexpr int YS = N2 + PZ;
@@ -480,6 +483,8 @@ struct SlabPipe {
 
     static FX_HD int nslabs(long bid, long nslab) { return Pencil<C>::nslabs(bid, nslab); }
 
+    // Points J0 .. J1-1 of each y-line (all of them by default).
+    template <int J0 = 0, int J1 = C::N1>
     static FX_HD void load_y(int tid, long b, long nslab, const cx<T>* __restrict__ in, Regs& R)
     {
         const int nl = nslabs(b, nslab) * C::N2;
@@ -492,7 +497,7 @@ struct SlabPipe {
             const int z = l - s * C::N2;
             const cx<T>* __restrict__ p = src + (long)s * C::PLANE + z;
 #pragma unroll
-            for (int j = 0; j < C::N1; ++j) R[i][j] = p[j * C::N2];
+            for (int j = J0; j < J1; ++j) R[i][j] = p[j * C::N2];
         }
     }
     static FX_HD void ypass(int tid, long b, long nslab, Regs& R, cx<T>* sm,
@@ -536,6 +541,7 @@ __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::W
     const int tid = threadIdx.x;
     long b = blockIdx.x;
     if (b >= nblocks) return;
+    if (C::ST && (blockIdx.x & 1)) __builtin_amdgcn_s_sleep(60);  // 60 x 64 cycles
     K::load_y(tid, b, nslab, in, R);
     for (;;) {
         long claim = 0;
@@ -544,8 +550,10 @@ __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::W
         if (tid == 0) s_next = claim;
         __syncthreads();
         const long nb = s_next;
-        if (nb < nblocks) K::load_y(tid, nb, nslab, in, R);
+        if (nb < nblocks) K::template load_y<0, C::SPLIT>(tid, nb, nslab, in, R);
         K::zpass(tid, b, nslab, sm, bk);
+        if (C::SPLIT < C::N1 && nb < nblocks)
+            K::template load_y<C::SPLIT, C::N1>(tid, nb, nslab, in, R);
         __syncthreads();
         K::store(tid, b, nslab, sm, out);
         if (nb >= nblocks) break;

@@ -1,7 +1,8 @@
 // Two-pass pencil kernels for volumes that exceed LDS at complex128 (L = 24, 36 and
 // 24x24x48) and their complex64 counterparts.  Names: s<SPW> slabs per workgroup,
 // t<NT> threads, b<NTB> threads of the x-line pass, c<CHUNK> volumes per launch pair
-// (0 = whole batch), p<P> persistent pipelined pass A with at most P workgroups per CU.
+// (0 = whole batch), p<P> persistent pipelined pass A with at most P workgroups per CU,
+// "_st" pass A with staggered HBM traffic (PencilCfg ST = 1).
 #include "kernels.hpp"
 
 namespace fx {
@@ -14,10 +15,20 @@ namespace fx {
     FX_P(N0, N1, N2, T, +1, SPW, NT, PZ, NTB, CH, PIPE,                                          \
          "pencil_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_s" #SPW "t" #NT "b" #NTB "c" #CH "p" #PIPE)
 
+#define FX_PS(N0, N1, N2, T, S, SPW, NT, PZ, NTB, CH, PIPE, NAME) \
+    r.push_back(pencil_desc<PencilCfg<N0, N1, N2, T, S, SPW, NT, PZ, NTB, CH, PIPE, 1>>(NAME))
+#define FX_PS_BOTH(N0, N1, N2, T, TAG, SPW, NT, PZ, NTB, CH, PIPE)                                \
+    FX_PS(N0, N1, N2, T, -1, SPW, NT, PZ, NTB, CH, PIPE,                                          \
+         "pencil_" #N0 "x" #N1 "x" #N2 "_" TAG "_fwd_s" #SPW "t" #NT "b" #NTB "c" #CH "p" #PIPE "_st"); \
+    FX_PS(N0, N1, N2, T, +1, SPW, NT, PZ, NTB, CH, PIPE,                                          \
+         "pencil_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_s" #SPW "t" #NT "b" #NTB "c" #CH "p" #PIPE "_st")
+
 void register_pencil(std::vector<KernelDesc>& r)
 {
     // L = 36: slab 36x36 = 20.7 KB complex128.  The race keeps an earlier variant unless a
-    // later one is >2% faster: s7t256p1 (72 us at batch 128) ahead of s3t128p2 (74 us).
+    // later one is >2% faster: at batch 128 s7t256p1 staggered (71 us), then unstaggered
+    // (73 us), ahead of s3t128p2 (73-74 us).
+    FX_PS_BOTH(36, 36, 36, double, "c128", 7, 256, 1, 128, 0, 1);
     FX_P_BOTH(36, 36, 36, double, "c128", 7, 256, 1, 128, 0, 1);
     FX_P_BOTH(36, 36, 36, double, "c128", 3, 128, 1, 128, 0, 2);
     FX_P_BOTH(36, 36, 36, double, "c128", 3, 128, 1, 128, 0, 0);

@@ -176,8 +176,14 @@ void emu_spipe(const cx<typename C::T>* src, cx<typename C::T>* dst, long nslab,
             cx<T>* sm = lds.data();
             emu_threads(C::NT, [&](int t) { K::ypass(t, b, nslab, R[w][t].r, sm, bk); });
             if (nb < nblocks)
-                emu_threads(C::NT, [&](int t) { K::load_y(t, nb, nslab, src, R[w][t].r); });
+                emu_threads(C::NT, [&](int t) {
+                    K::template load_y<0, C::SPLIT>(t, nb, nslab, src, R[w][t].r);
+                });
             emu_threads(C::NT, [&](int t) { K::zpass(t, b, nslab, sm, bk); });
+            if (C::SPLIT < C::N1 && nb < nblocks)
+                emu_threads(C::NT, [&](int t) {
+                    K::template load_y<C::SPLIT, C::N1>(t, nb, nslab, src, R[w][t].r);
+                });
             emu_threads(C::NT, [&](int t) { K::store(t, b, nslab, sm, dst); });
             if (!lds.guards_ok() && g_emu_bad_block < 0) g_emu_bad_block = b;
         });
