@@ -292,13 +292,21 @@ struct FusedPipe {
     }
 };
 
+// Start of a staggered persistent workgroup: those with bit 3 of the index set wait ~1.6 us.
+// Workgroups are placed on the 8 XCDs round-robin by index, so this delays half of the CUs
+// of every XCD (delaying whole XCDs, odd indices, measured 1 us slower at 17^3).
+__device__ __forceinline__ void stagger_start()
+{
+    if ((blockIdx.x >> 3) & 1) __builtin_amdgcn_s_sleep(60);  // 60 x 64 cycles
+}
+
 // A pipelined kernel launched with at most P workgroups per CU (P > 0) may spend the
 // registers of P resident workgroups rather than of as many as the LDS footprint allows.
 // ST = 1 staggers the HBM traffic: the next block's x-lines are issued in two halves (the
-// first ceil(N0/2) points after the x pass, the rest after the z pass) and odd workgroups
-// start ~1.6 us late, so the CUs' read and write phases spread out instead of every CU
-// reading at once.  17^3 complex128 at batch 1024: 38.1 us against 40.2 us; kernels of
-// ~15 us (8^3, 6^3, 8x8x16) lose more to the late start than they gain.
+// first ceil(N0/2) points after the x pass, the rest after the z pass) and half of the
+// workgroups start late (stagger_start), so the CUs' read and write phases spread out
+// instead of every CU reading at once.  17^3 complex128 at batch 1024: 36.3 us against
+// 38.5 us; kernels of ~15 us (8^3, 6^3, 8x8x16) lose more to the late start than they gain.
 template <class C, int P, int ST_ = 0>
 struct PipeCfg : C {
     static constexpr int WPE =
@@ -333,7 +341,7 @@ __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::W
     const int tid = threadIdx.x;
     long b = blockIdx.x;
     if (b >= nblocks) return;
-    if (C::ST && (blockIdx.x & 1)) __builtin_amdgcn_s_sleep(60);  // 60 x 64 cycles
+    if (C::ST) stagger_start();
     K::load_x(tid, b, nvol, in, R);
     for (;;) {
         long claim = 0;
@@ -358,7 +366,7 @@ __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::W
 // PIPE = 0: pass A is k_slab (one slab group per workgroup); PIPE = P > 0: pass A is the
 // persistent pipelined k_spipe with at most P resident workgroups per CU.
 // ST = 1: k_spipe staggers its HBM traffic as PipeCfg ST does (y-lines of the next group
-// issued in two halves, odd workgroups starting ~1.6 us late).
+// issued in two halves, half of the workgroups starting ~1.6 us late).
 template <int N0_, int N1_, int N2_, typename T_, int S_, int SPW_, int NT_, int PZ_,
           int NTB_, int CHUNK_, int PIPE_ = 0, int ST_ = 0>
 struct PencilCfg {
@@ -541,7 +549,7 @@ __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::W
     const int tid = threadIdx.x;
     long b = blockIdx.x;
     if (b >= nblocks) return;
-    if (C::ST && (blockIdx.x & 1)) __builtin_amdgcn_s_sleep(60);  // 60 x 64 cycles
+    if (C::ST) stagger_start();
     K::load_y(tid, b, nslab, in, R);
     for (;;) {
         long claim = 0;
