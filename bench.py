@@ -40,9 +40,9 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the headline kernel (pipe_17x17x17_c128_fwd_v1t320p1_fold_st, the
 # race's pick) from rocprofv3 FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on the
-# runtime's copy of a known size: 39,438.8 KiB x 2 + 78,667.9 KiB
+# runtime's copy of a known size: 39,440.0 KiB x 2 + 78,659.4 KiB
 # (profiles/k_fpipe_17x17x17_c128_fwd_b1024.md).
-TRAFFIC = int((39438.8 * 2 + 78667.9) * 1024)
+TRAFFIC = int((39440.0 * 2 + 78659.4) * 1024)
 TRAFFIC_KERNEL = "pipe_17x17x17_c128_fwd_v1t320p1_fold_st"
 ROTATE_BYTES = 2 << 30
 
