@@ -42,6 +42,9 @@ void register_pencil(std::vector<KernelDesc>& r)
     FX_P_BOTH(24, 24, 24, float, "c64", 16, 384, 1, 128, 0, 2);
     FX_P_BOTH(24, 24, 24, float, "c64", 16, 384, 1, 128, 0, 0);
     // 24 x 24 x 48: slab 24x48 = 18.4 KB complex128; z lines 48-point, y and x 24-point.
+    // Two slabs per 128-thread block first (768 blocks at batch 64, ~3 resident per CU, so
+    // blocks in different phases overlap): 26.6 us against 27.9 us for four slabs.
+    FX_P_BOTH(24, 24, 48, double, "c128", 2, 128, 1, 128, 0, 0);
     FX_P_BOTH(24, 24, 48, double, "c128", 4, 128, 1, 128, 0, 0);
     FX_P_BOTH(24, 24, 48, double, "c128", 8, 256, 1, 128, 0, 0);
     FX_P_BOTH(24, 24, 48, float, "c64", 8, 384, 1, 128, 0, 2);
