@@ -14,6 +14,13 @@
 //   wave covers consecutive (y,z), i.e. contiguous addresses, so no LDS is needed at all.
 //   Pass B re-reads what pass A wrote; chunking keeps that intermediate cache resident.
 //
+// Persistent pipelined forms (k_fpipe for fused geometries, k_spipe for pencil pass A): one
+// or a few workgroups per CU claim blocks from a per-launch counter (claim_block); a
+// block's first line pass reads its lines straight from HBM into VGPRs, and the next
+// block's lines are issued as soon as those registers are free, so they load while the
+// current block is in its remaining passes.  PipeCfg / PencilCfg ST = 1 additionally
+// spreads each workgroup's HBM requests over the iteration (stagger_start).
+//
 // Each kernel body is a sequence of phase functions separated by workgroup barriers.  A
 // phase only reads LDS locations written in an earlier phase or by the same thread, so the
 // phases are plain __host__ __device__ functions of (thread, block).  Launchers come from
