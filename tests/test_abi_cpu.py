@@ -145,8 +145,13 @@ def test_create_rejects_bad_arguments(pkg):
         assert not lib.fft3d_mx_create(*args)
         assert word in lib.fft3d_mx_last_error().decode()
     assert not lib.fft3d_mx_create_named(8, 8, 8, 4, -1, 0, 0, b"")
-    assert not lib.fft3d_mx_create_named(8, 8, 8, 4, -1, 0, 0, b"pipe_17x17x17_c128_fwd_v1t320p2_fold")
-    assert "does not serve" in lib.fft3d_mx_last_error().decode()
+    # a registered kernel of another geometry, and a name no kernel has
+    other = b"pipe_17x17x17_c128_fwd_v1t320p1_fold"
+    assert other.decode() in [k[0] for k in pkg.fft3d.kernels()]
+    for name in (other, b"no_such_kernel"):
+        lib.fft3d_mx_clear_error()
+        assert not lib.fft3d_mx_create_named(8, 8, 8, 4, -1, 0, 0, name)
+        assert "does not serve" in lib.fft3d_mx_last_error().decode()
     assert lib.fft3d_mx_execute(None, None, None, None) == -1
     assert lib.fft3d_mx_plan_get_info(None, None) == -1
     assert not lib.fft3d_create(7, 1) and not lib.fft3d_gpu_create(7, 1)
