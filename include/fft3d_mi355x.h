@@ -74,7 +74,9 @@ fft3d_mx_plan *fft3d_mx_create_named(int n0, int n1, int n2, int batch, int dire
                                      int precision, unsigned flags, const char *kernel);
 
 /* Enqueue the transform of the plan's batch on `stream` (a hipStream_t; NULL = the NULL
- * stream).  Asynchronous.  0 or a negative error code. */
+ * stream).  Asynchronous.  0 or a negative error code.  `in` and `out` are distinct device
+ * buffers aligned to the element size (16 B complex128, 8 B complex64; hipMalloc gives
+ * 256 B), as are the buffers of every other entry point below. */
 int fft3d_mx_execute(fft3d_mx_plan *plan, const void *in, void *out, void *stream);
 
 /* Same for the first `batch` volumes only (0 <= batch <= the planned batch). */

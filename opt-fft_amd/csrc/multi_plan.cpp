@@ -248,6 +248,10 @@ extern "C" int fft3d_mx_multi_execute(fft3d_mx_multi* m, const void* const* in, 
         if (m->batch[i] == 0) continue;
         if (!in[i] || !out[i] || in[i] == (const void*)out[i])
             return fail(FFT3D_MX_EINVAL, "fft3d_mx_multi_execute: item %d needs distinct buffers", i);
+        if (!elem_aligned(in[i], m->prec) || !elem_aligned(out[i], m->prec))
+            return fail(FFT3D_MX_EINVAL,
+                        "fft3d_mx_multi_execute: item %d buffers must be aligned to %zu bytes", i,
+                        elem_bytes(m->prec));
     }
     int cur = m->device;
     (void)hipGetDevice(&cur);

@@ -387,6 +387,9 @@ extern "C" int fft3d_mx_execute_n(fft3d_mx_plan* p, const void* in, void* out, l
     if (!in || !out) return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute: NULL buffer");
     if (in == (const void*)out)
         return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute: in and out must be distinct");
+    if (!elem_aligned(in, p->prec) || !elem_aligned(out, p->prec))
+        return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute: buffers must be aligned to %zu bytes",
+                    elem_bytes(p->prec));
     int cur = p->device;
     (void)hipGetDevice(&cur);
     if (cur != p->device) (void)hipSetDevice(p->device);
@@ -411,6 +414,10 @@ extern "C" int fft3d_mx_execute_chain(fft3d_mx_plan* p, const void* in, void* ou
     if (m < 1) return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute_chain: m = %d < 1", m);
     if (!in || !out || !pong || in == (const void*)out || in == (const void*)pong || out == pong)
         return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute_chain: in, out, pong must be distinct");
+    if (!elem_aligned(in, p->prec) || !elem_aligned(out, p->prec) || !elem_aligned(pong, p->prec))
+        return fail(FFT3D_MX_EINVAL,
+                    "fft3d_mx_execute_chain: buffers must be aligned to %zu bytes",
+                    elem_bytes(p->prec));
     if (!(step_scale == step_scale))
         return fail(FFT3D_MX_EINVAL, "fft3d_mx_execute_chain: step_scale is NaN");
     int cur = p->device;

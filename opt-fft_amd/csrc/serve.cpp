@@ -514,6 +514,9 @@ extern "C" long fft3d_srv_submit(fft3d_srv* s, const fft3d_srv_request* r, doubl
     if (r->batch < 1 || r->batch > kMaxBatch || !r->in ||
         (r->direction != -1 && r->direction != 1) || (r->precision != 0 && r->precision != 1))
         return fail(FFT3D_MX_EINVAL, "fft3d_srv_submit: bad request");
+    if (!elem_aligned(r->in, r->precision) || (r->out && !elem_aligned(r->out, r->precision)))
+        return fail(FFT3D_MX_EINVAL, "fft3d_srv_submit: buffers must be aligned to %zu bytes",
+                    elem_bytes(r->precision));
     if (!fft3d_mx_supports(r->n0, r->n1, r->n2, r->precision))
         return fail(FFT3D_MX_EUNSUPPORTED, "fft3d_srv_submit: %dx%dx%d not supported", r->n0,
                     r->n1, r->n2);

@@ -170,6 +170,16 @@ def test_execute_errors_are_reported(gpu):
         plan.execute(0, 0)
     with pytest.raises(RuntimeError):
         plan.execute(1 << 20, 1 << 21, batch=5)
+    # misaligned buffers are refused before anything is launched
+    h = gpu.hip
+    bi, bo = h.DeviceBuffer(8 * 512 * 16 + 64), h.DeviceBuffer(8 * 512 * 16 + 64)
+    for a, b in ((bi.addr + 8, bo.addr), (bi.addr, bo.addr + 8)):
+        with pytest.raises(RuntimeError, match="aligned"):
+            plan.execute(a, b)
+    plan.execute(bi.addr + 16, bo.addr + 16)  # element-aligned offsets are fine
+    h.synchronize()
+    bi.close()
+    bo.close()
     plan.close()
     with pytest.raises(ValueError):
         f.Plan((7, 7, 7), 1)
