@@ -38,12 +38,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch of the headline kernel (pipe_17x17x17_c128_fwd_v1t320p1_fold_st, the
+# HBM bytes per launch of the headline kernel (pipe_17x17x17_c128_fwd_v2t640p1_fold_st3, the
 # race's pick) from rocprofv3 FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on the
-# runtime's copy of a known size: 39,440.0 KiB x 2 + 78,659.4 KiB
+# runtime's copy of a known size: 40,011.1 KiB x 2 + 78,839.9 KiB
 # (profiles/k_fpipe_17x17x17_c128_fwd_b1024.md).
-TRAFFIC = int((39440.0 * 2 + 78659.4) * 1024)
-TRAFFIC_KERNEL = "pipe_17x17x17_c128_fwd_v1t320p1_fold_st"
+TRAFFIC = int((40011.0625 * 2 + 78839.90625) * 1024)
+TRAFFIC_KERNEL = "pipe_17x17x17_c128_fwd_v2t640p1_fold_st3"
 ROTATE_BYTES = 2 << 30
 
 HEADLINE = dict(name="L17_c128_fwd_b1024", shape=(17, 17, 17), batch=1024, dist="uniform",

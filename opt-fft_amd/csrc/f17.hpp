@@ -67,7 +67,7 @@ struct F17 {
         for (int u = 0; u < U; ++u) {
             int e = t + u * NT;
             e = e < V ? e : V - 1;
-            R.r[u] = src[e];
+            R.r[u] = ld_stream(src + e);
         }
     }
 
@@ -113,7 +113,7 @@ struct F17 {
             fold17_put<C::S>(x, bk, [&](int k, const cx<T>& y) { sm[b + k * st] = y; });
         } else {
             fold17_put<C::S>(x, bk, [&](int k, const cx<T>& y) {
-                dst[b + k * st] = cx<T>{y.r * scl, y.i * scl};
+                st_stream(dst + b + k * st, cx<T>{y.r * scl, y.i * scl});
             });
         }
     }
@@ -165,9 +165,9 @@ struct F17 {
             sm[b + (17 - p.k) * st] = p.ykm;
             if (p.k == 1) sm[b] = p.y0;
         } else {
-            dst[b + p.k * st] = cx<T>{p.yk.r * scl, p.yk.i * scl};
-            dst[b + (17 - p.k) * st] = cx<T>{p.ykm.r * scl, p.ykm.i * scl};
-            if (p.k == 1) dst[b] = cx<T>{p.y0.r * scl, p.y0.i * scl};
+            st_stream(dst + b + p.k * st, cx<T>{p.yk.r * scl, p.yk.i * scl});
+            st_stream(dst + b + (17 - p.k) * st, cx<T>{p.ykm.r * scl, p.ykm.i * scl});
+            if (p.k == 1) st_stream(dst + b, cx<T>{p.y0.r * scl, p.y0.i * scl});
         }
     }
 };

@@ -70,7 +70,7 @@ hipError_t launch_fused(const void* in, void* out, long nvol, const void* bank, 
     return hipGetLastError();
 }
 
-template <class C, int PERSIST, int ST = 0>
+template <class C, int PERSIST, int ST = 0, int HN = 3>
 hipError_t launch_fpipe(const void* in, void* out, long nvol, const void* bank, double scl,
                         unsigned* work, hipStream_t stream)
 {
@@ -78,7 +78,7 @@ hipError_t launch_fpipe(const void* in, void* out, long nvol, const void* bank, 
     if (nvol <= 0) return hipSuccess;
     const long nblocks = (nvol + C::VPW - 1) / C::VPW;
     if (nblocks >= (1L << 31)) return hipErrorInvalidValue;
-    using P = PipeCfg<C, PERSIST, ST>;
+    using P = PipeCfg<C, PERSIST, ST, HN>;
     const long grid = PERSIST > 0 ? persistent_grid(k_fpipe<P>, C::NT, PERSIST, nblocks) : nblocks;
     hipLaunchKernelGGL((k_fpipe<P>), dim3((unsigned)grid), dim3(C::NT), 0, stream,
                        (const cx<T>*)in, (cx<T>*)out, nvol, nblocks, (const Bank<T>*)bank,
@@ -125,12 +125,12 @@ KernelDesc fused_desc(const char* name)
                       &launch_fused<C>};
 }
 
-template <class C, int PERSIST, int ST = 0>
+template <class C, int PERSIST, int ST = 0, int HN = 3>
 KernelDesc fpipe_desc(const char* name)
 {
     return KernelDesc{C::N0, C::N1, C::N2, prec_of<typename C::T>(), C::S, kFused, C::VPW,
                       C::NT, C::V17, PERSIST, sizeof(cx<typename C::T>) * (size_t)C::LDS, name,
-                      &launch_fpipe<C, PERSIST, ST>};
+                      &launch_fpipe<C, PERSIST, ST, HN>};
 }
 
 template <class C>

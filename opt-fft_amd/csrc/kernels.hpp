@@ -37,7 +37,7 @@ constexpr int prec_of() { return sizeof(T) == 8 ? kC128 : kC64; }
 // Copy TOT contiguous elements HBM -> LDS, slot(e) giving the padded LDS index.  Loads
 // are issued in groups of up to 16 per thread before any LDS write, so each wave keeps
 // 16 x 1 KiB (complex128) of reads in flight.
-template <int TOT, int NT, typename T, typename SlotFn>
+template <int TOT, int NT, int HL = 1, typename T, typename SlotFn>
 FX_HD void stage_full(int tid, const cx<T>* __restrict__ src, cx<T>* sm, SlotFn slot)
 {
     constexpr int ITER = (TOT + NT - 1) / NT;
@@ -50,7 +50,7 @@ FX_HD void stage_full(int tid, const cx<T>* __restrict__ src, cx<T>* sm, SlotFn 
 #pragma unroll
         for (int k = 0; k < GROUP; ++k) {
             const int e = tid + (g + k) * NT;
-            if (g + k < ITER) buf[k] = src[(TOT % NT == 0 || e < TOT) ? e : TOT - 1];
+            if (g + k < ITER) buf[k] = ld_hint<HL>(src + ((TOT % NT == 0 || e < TOT) ? e : TOT - 1));
         }
 #pragma unroll
         for (int k = 0; k < GROUP; ++k) {
@@ -100,6 +100,7 @@ struct FusedCfg {
     static constexpr int VS = N0 * XS;           // LDS pitch between volumes
     static constexpr int LDS = VPW * VS;         // elements
     static constexpr int WPE = waves_per_eu(LDS * (int)sizeof(cx<T>), NT, 0);
+    static constexpr int HN = 3;                 // streaming hints: 1 loads, 2 stores
 };
 
 template <class C>
@@ -128,9 +129,9 @@ struct Fused {
         const int nv = nvols(bid, nvol);
         const cx<T>* __restrict__ src = in + bid * C::VPW * (long)C::V;
         if (nv == C::VPW)
-            stage_full<C::VPW * C::V, C::NT>(tid, src, sm, [](int e) { return slot(e); });
+            stage_full<C::VPW * C::V, C::NT, C::HN & 1>(tid, src, sm, [](int e) { return slot(e); });
         else
-            for (int e = tid; e < nv * C::V; e += C::NT) sm[slot(e)] = src[e];
+            for (int e = tid; e < nv * C::V; e += C::NT) sm[slot(e)] = ld_hint<C::HN & 1>(src + e);
     }
     // Phase 2: z lines (v, x, y), contiguous in LDS.
     static FX_HD void zpass(int tid, long bid, long nvol, cx<T>* sm, const Bank<T>* __restrict__ bk)
@@ -180,10 +181,11 @@ struct Fused {
             cx<T>* __restrict__ o = dst + (long)v * C::V + q;
             if (scaled) {
 #pragma unroll
-                for (int k = 0; k < C::N0; ++k) o[k * C::PLANE] = scale(t[k], scl);
+                for (int k = 0; k < C::N0; ++k)
+                    st_hint<C::HN & 2>(o + k * C::PLANE, scale(t[k], scl));
             } else {
 #pragma unroll
-                for (int k = 0; k < C::N0; ++k) o[k * C::PLANE] = t[k];
+                for (int k = 0; k < C::N0; ++k) st_hint<C::HN & 2>(o + k * C::PLANE, t[k]);
             }
         }
     }
@@ -243,7 +245,7 @@ struct FusedPipe {
             const int q = l - v * C::PLANE;
             const cx<T>* __restrict__ p = src + (long)v * C::V + q;
 #pragma unroll
-            for (int j = J0; j < J1; ++j) R[i][j] = p[j * C::PLANE];
+            for (int j = J0; j < J1; ++j) R[i][j] = ld_hint<C::HN & 1>(p + j * C::PLANE);
         }
     }
     // x DFT in registers, results to their LDS slots.
@@ -287,14 +289,20 @@ struct FusedPipe {
 #pragma unroll
             for (int j = 0; j < C::N1; ++j) t[j] = sm[s0 + j * C::YS];
             cx<T>* __restrict__ o = dst + (long)v * C::V + x * C::PLANE + z;
+            // Hinted only where a z run fills whole 128 B lines, so that every store
+            // instruction of a wave writes complete lines; shorter runs are completed by
+            // later instructions and lose with the streaming policy.
+            constexpr int H = (C::HN & 2) && (C::N2 * (int)sizeof(cx<T>)) % 128 == 0;
             if (cj)
                 dft_put<C::N1, C::S, T, C::V17>(t, bk, [&](int k, cx<T> w) {
-                    o[k * C::N2] = cx<T>{w.r * scl, -(w.i * scl)};
+                    st_hint<H>(o + k * C::N2, cx<T>{w.r * scl, -(w.i * scl)});
                 });
             else if (scaled)
-                dft_put<C::N1, C::S, T, C::V17>(t, bk, [&](int k, cx<T> w) { o[k * C::N2] = scale(w, scl); });
+                dft_put<C::N1, C::S, T, C::V17>(
+                    t, bk, [&](int k, cx<T> w) { st_hint<H>(o + k * C::N2, scale(w, scl)); });
             else
-                dft_put<C::N1, C::S, T, C::V17>(t, bk, [&](int k, cx<T> w) { o[k * C::N2] = w; });
+                dft_put<C::N1, C::S, T, C::V17>(
+                    t, bk, [&](int k, cx<T> w) { st_hint<H>(o + k * C::N2, w); });
         }
     }
 };
@@ -314,12 +322,28 @@ __device__ __forceinline__ void stagger_start()
 // workgroups start late (stagger_start), so the CUs' read and write phases spread out
 // instead of every CU reading at once.  17^3 complex128 at batch 1024: 36.3 us against
 // 38.5 us; kernels of ~15 us (8^3, 6^3, 8x8x16) lose more to the late start than they gain.
-template <class C, int P, int ST_ = 0>
+// ST = 2 issues the next block's x-lines in thirds (after the x, the z and the y pass), so
+// that only two thirds of them occupy VGPRs during the y pass: the two-volume 640-thread
+// 17^3 kernel (168 VGPRs at 10 waves per CU) otherwise spills two prefetched points and
+// waits for their loads to land before spilling them.
+template <class C, int P, int ST_ = 0, int HN_ = 3>
 struct PipeCfg : C {
     static constexpr int WPE =
         P > 0 ? waves_per_eu(C::LDS * (int)sizeof(cx<typename C::T>), C::NT, P) : C::WPE;
     static constexpr int ST = ST_;
-    static constexpr int SPLIT = ST_ ? (C::N0 + 1) / 2 : C::N0;
+    static constexpr int HN = HN_;
+    // x-line points [0, SPLIT) are issued after the x pass, [SPLIT, SPLIT2) after the z
+    // pass, [SPLIT2, N0) after the y pass.
+    static constexpr int SPLIT = ST_ == 1 ? (C::N0 + 1) / 2 : ST_ == 2 ? (C::N0 + 2) / 3 : C::N0;
+    static constexpr int SPLIT2 = ST_ == 2 ? 2 * ((C::N0 + 2) / 3) : C::N0;
+};
+
+// C with the streaming hints off.  The complex64 mixed-geometry launch (6^3 .. 17^3 blocks
+// through one persistent kernel, multi.hpp) measured 221 us with hinted x-line loads against
+// 207-209 us with plain ones on the same device.
+template <class C>
+struct PlainCfg : C {
+    static constexpr int HN = 0;
 };
 
 // Next block of a persistent workgroup.  Each workgroup starts at block blockIdx.x and then
@@ -359,11 +383,12 @@ __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::W
         const long nb = s_next;
         if (nb < nblocks) K::template load_x<0, C::SPLIT>(tid, nb, nvol, in, R);
         K::zpass(tid, b, nvol, sm, bk);
-        if (C::SPLIT < C::N0 && nb < nblocks)
-            K::template load_x<C::SPLIT, C::N0>(tid, nb, nvol, in, R);
+        if (C::SPLIT < C::SPLIT2 && nb < nblocks)
+            K::template load_x<C::SPLIT, C::SPLIT2>(tid, nb, nvol, in, R);
         __syncthreads();
         K::ypass(tid, b, nvol, sm, out, bk, scl);
         if (nb >= nblocks) break;
+        if (C::SPLIT2 < C::N0) K::template load_x<C::SPLIT2, C::N0>(tid, nb, nvol, in, R);
         __syncthreads();
         b = nb;
     }
@@ -414,7 +439,7 @@ struct Pencil {
         if (ns == C::SPW)
             stage_full<C::SPW * C::PLANE, C::NT>(tid, src, sm, [](int e) { return slot(e); });
         else
-            for (int e = tid; e < ns * C::PLANE; e += C::NT) sm[slot(e)] = src[e];
+            for (int e = tid; e < ns * C::PLANE; e += C::NT) sm[slot(e)] = ld_stream(src + e);
     }
     // Pass A, phase 2: z lines (s, y).
     static FX_HD void zpass(int tid, long bid, long nslab, cx<T>* sm, const Bank<T>* __restrict__ bk)
@@ -461,10 +486,10 @@ struct Pencil {
         dft<C::N0, C::S, T>(t, bk);
         if (scl != T(1)) {
 #pragma unroll
-            for (int k = 0; k < C::N0; ++k) p[k * C::PLANE] = scale(t[k], scl);
+            for (int k = 0; k < C::N0; ++k) st_stream(p + k * C::PLANE, scale(t[k], scl));
         } else {
 #pragma unroll
-            for (int k = 0; k < C::N0; ++k) p[k * C::PLANE] = t[k];
+            for (int k = 0; k < C::N0; ++k) st_stream(p + k * C::PLANE, t[k]);
         }
     }
 };
@@ -512,7 +537,7 @@ struct SlabPipe {
             const int z = l - s * C::N2;
             const cx<T>* __restrict__ p = src + (long)s * C::PLANE + z;
 #pragma unroll
-            for (int j = J0; j < J1; ++j) R[i][j] = p[j * C::N2];
+            for (int j = J0; j < J1; ++j) R[i][j] = ld_stream(p + j * C::N2);
         }
     }
     static FX_HD void ypass(int tid, long b, long nslab, Regs& R, cx<T>* sm,

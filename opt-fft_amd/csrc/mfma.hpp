@@ -217,7 +217,7 @@ __device__ __forceinline__ void mfma_block(long bid, long nvol, const cx<float>*
     const float ci = conj ? -1.f : 1.f;
     for (int e = tid; e < nv * C::V; e += C::NT) {
         const cx<float> y = sm[K::slot(e)];
-        dst[e] = cx<float>{y.r, ci * y.i};
+        st_stream(dst + e, cx<float>{y.r, ci * y.i});
     }
     __syncthreads();
 }

@@ -79,7 +79,7 @@ template <class C>
 FX_HD void multi_phase(int ph, int tid, long b, const MultiItem<typename C::T>& it,
                        cx<typename C::T>* sm, const Bank<typename C::T>* __restrict__ bk)
 {
-    using K = FusedPipe<C>;
+    using K = FusedPipe<PlainCfg<C>>;
     if (ph == 0) {
         typename K::Regs R;
         K::load_x(tid, b, it.nvol, it.in, R);

@@ -118,7 +118,7 @@ hipError_t launch_fused(const void* in, void* out, long nvol, const void* bank, 
 // previous block's LDS contents).
 constexpr long kEmuPersistentGrid = 3;
 
-template <class C, int PERSIST, int ST = 0>
+template <class C, int PERSIST, int ST = 0, int HN = 3>
 hipError_t launch_fpipe(const void* in, void* out, long nvol, const void* bank, double scl,
                         unsigned* work, hipStream_t)
 {
@@ -141,17 +141,22 @@ hipError_t launch_fpipe(const void* in, void* out, long nvol, const void* bank, 
             lds.poison();
             cx<T>* sm = lds.data();
             constexpr int SPLIT = PipeCfg<C, PERSIST, ST>::SPLIT;
+            constexpr int SPLIT2 = PipeCfg<C, PERSIST, ST>::SPLIT2;
             emu_threads(C::NT, [&](int t) { K::xpass(t, b, nvol, R[w][t].r, sm, bk); });
             if (nb < nblocks)
                 emu_threads(C::NT, [&](int t) {
                     K::template load_x<0, SPLIT>(t, nb, nvol, src, R[w][t].r);
                 });
             emu_threads(C::NT, [&](int t) { K::zpass(t, b, nvol, sm, bk); });
-            if (SPLIT < C::N0 && nb < nblocks)
+            if (SPLIT < SPLIT2 && nb < nblocks)
                 emu_threads(C::NT, [&](int t) {
-                    K::template load_x<SPLIT, C::N0>(t, nb, nvol, src, R[w][t].r);
+                    K::template load_x<SPLIT, SPLIT2>(t, nb, nvol, src, R[w][t].r);
                 });
             emu_threads(C::NT, [&](int t) { K::ypass(t, b, nvol, sm, dst, bk, (T)scl); });
+            if (SPLIT2 < C::N0 && nb < nblocks)
+                emu_threads(C::NT, [&](int t) {
+                    K::template load_x<SPLIT2, C::N0>(t, nb, nvol, src, R[w][t].r);
+                });
             if (!lds.guards_ok() && g_emu_bad_block < 0) g_emu_bad_block = b;
         });
     return hipSuccess;
@@ -230,12 +235,12 @@ KernelDesc fused_desc(const char* name)
                       &launch_fused<C>};
 }
 
-template <class C, int PERSIST, int ST = 0>
+template <class C, int PERSIST, int ST = 0, int HN = 3>
 KernelDesc fpipe_desc(const char* name)
 {
     return KernelDesc{C::N0, C::N1, C::N2, prec_of<typename C::T>(), C::S, kFused, C::VPW,
                       C::NT, C::V17, PERSIST, sizeof(cx<typename C::T>) * (size_t)C::LDS, name,
-                      &launch_fpipe<C, PERSIST, ST>};
+                      &launch_fpipe<C, PERSIST, ST, HN>};
 }
 
 template <class C>
