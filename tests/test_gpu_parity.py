@@ -91,6 +91,34 @@ def test_workload_sizes_forward_and_inverse(gpu, shape, batch, dist):
     bwd.close()
 
 
+def test_pipelined_17_variants_over_several_blocks(gpu):
+    """Every persistent 17^3 variant, forced, at a batch where each workgroup runs several
+    blocks (the prefetch of the next block, in halves or thirds, and the claimed walk run
+    only then; at batch 1 and 5 each workgroup holds a single block)."""
+    f = gpu.fft3d
+    shape = (17, 17, 17)
+    names = [k for k in _kernels(gpu) if k[0].startswith("pipe_17x17x17_")]
+    assert any(k[0].endswith("_st3") for k in names)
+    ref = {}
+    for name, _, prec, direction in names:
+        dt = np.complex128 if prec == f.C128 else np.complex64
+        batch = 1024 if prec == f.C128 else 2048
+        if (prec, direction) not in ref:
+            x = oracle.make_input(shape, batch, seed=77, dist="uniform", dtype=dt)
+            ref[(prec, direction)] = (x, oracle.fftn_numpy(x, sign=direction))
+        x, want = ref[(prec, direction)]
+        print(f"{name} B={batch} ...", end=" ", flush=True)
+        plan = f.Plan(shape, batch, direction, prec, kernel=name)
+        assert plan.kernel == name
+        y = plan.run(x)
+        y2 = plan.run(x)
+        plan.close()
+        err = oracle.rel_l2(y, want)
+        print(f"rel_l2={err:.2e}", flush=True)
+        assert err <= TOL[prec], f"{name}: rel_l2 {err:.3e}"
+        assert np.array_equal(y, y2), f"{name}: second execute differs"
+
+
 def test_repeatable_and_input_untouched(gpu):
     f = gpu.fft3d
     h = gpu.hip
