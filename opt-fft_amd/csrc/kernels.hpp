@@ -482,7 +482,7 @@ struct Pencil {
         cx<T>* __restrict__ p = buf + v * C::V + q;
         cx<T> t[C::N0];
 #pragma unroll
-        for (int j = 0; j < C::N0; ++j) t[j] = p[j * C::PLANE];
+        for (int j = 0; j < C::N0; ++j) t[j] = ld_stream(p + j * C::PLANE);
         dft<C::N0, C::S, T>(t, bk);
         if (scl != T(1)) {
 #pragma unroll
