@@ -19,7 +19,13 @@
 // block's first line pass reads its lines straight from HBM into VGPRs, and the next
 // block's lines are issued as soon as those registers are free, so they load while the
 // current block is in its remaining passes.  PipeCfg / PencilCfg ST = 1 additionally
-// spreads each workgroup's HBM requests over the iteration (stagger_start).
+// spreads each workgroup's HBM requests over the iteration (next lines in two halves,
+// stagger_start); PipeCfg ST = 2 issues them in thirds, the last after the y pass, so fewer
+// prefetched registers are live during it.
+//
+// HBM accesses that touch an element once use the nontemporal forms of cx.hpp (ld_hint /
+// st_hint, HN in the configs): inputs, outputs written in whole lines, and the pencil
+// intermediate as pass B reads it.
 //
 // Each kernel body is a sequence of phase functions separated by workgroup barriers.  A
 // phase only reads LDS locations written in an earlier phase or by the same thread, so the
