@@ -18,10 +18,11 @@ namespace fx {
     FX_17P(T, -1, VPW, NT, ALG, P, "pipe_17x17x17_" TAG "_fwd_v" #VPW "t" #NT "p" #P "_" ATAG); \
     FX_17P(T, +1, VPW, NT, ALG, P, "pipe_17x17x17_" TAG "_bwd_v" #VPW "t" #NT "p" #P "_" ATAG)
 // Staggered HBM traffic (PipeCfg ST = 1): suffix "_st"; ST = 2, the next block's x-lines
-// issued in thirds: suffix "_st3".  HN = streaming hints (1 loads, 2 stores; the pipelined
-// y pass never hints its 17-element store runs).  Two volumes per 640-thread workgroup with
-// hinted loads, in thirds (155 VGPRs, no spill): 36.3-37.5 us at batch 1024 against
-// 39.2-40.2 us for one volume per 320-thread workgroup, staggered, on the same device.
+// issued in three parts: suffix "_st3".  HN = streaming hints (1 loads, 2 stores; the
+// pipelined y pass never hints its 17-element store runs).  Two volumes per 640-thread
+// workgroup with hinted loads, in three parts (161 VGPRs, no spill): 36.3-37.5 us at batch
+// 1024 against 39.2-40.2 us for one volume per 320-thread workgroup, staggered, on the same
+// device.
 #define FX_17S(T, S, VPW, NT, ALG, P, ST, HN, NAME) \
     r.push_back(fpipe_desc<FusedCfg<17, 17, 17, T, S, VPW, NT, 0, 0, ALG>, P, ST, HN>(NAME))
 #define FX_17S_BOTH(T, TAG, VPW, NT, ALG, ATAG, P, ST, HN, SFX)                                \

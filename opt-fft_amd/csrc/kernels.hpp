@@ -20,8 +20,8 @@
 // block's lines are issued as soon as those registers are free, so they load while the
 // current block is in its remaining passes.  PipeCfg / PencilCfg ST = 1 additionally
 // spreads each workgroup's HBM requests over the iteration (next lines in two halves,
-// stagger_start); PipeCfg ST = 2 issues them in thirds, the last after the y pass, so fewer
-// prefetched registers are live during it.
+// stagger_start); PipeCfg ST = 2 issues them in three parts, the last after the y pass, so
+// fewer prefetched registers are live during it.
 //
 // HBM accesses that touch an element once use the nontemporal forms of cx.hpp (ld_hint /
 // st_hint, HN in the configs): inputs, outputs written in whole lines, and the pencil
@@ -328,10 +328,12 @@ __device__ __forceinline__ void stagger_start()
 // workgroups start late (stagger_start), so the CUs' read and write phases spread out
 // instead of every CU reading at once.  17^3 complex128 at batch 1024: 36.3 us against
 // 38.5 us; kernels of ~15 us (8^3, 6^3, 8x8x16) lose more to the late start than they gain.
-// ST = 2 issues the next block's x-lines in thirds (after the x, the z and the y pass), so
-// that only two thirds of them occupy VGPRs during the y pass: the two-volume 640-thread
-// 17^3 kernel (168 VGPRs at 10 waves per CU) otherwise spills two prefetched points and
-// waits for their loads to land before spilling them.
+// ST = 2 issues the next block's x-lines in three parts (after the x, the z and the y
+// pass: points [0, ceil(N0/2)), up to N0 - ceil(N0/6), the rest), so that the last part
+// does not occupy VGPRs during the y pass: the two-volume 640-thread 17^3 kernel (168 VGPRs
+// at 10 waves per CU) otherwise spills two prefetched points and waits for their loads to
+// land before spilling them.  Split 9/5/3 at N0 = 17 (161 VGPRs): 36.5-36.9 us against
+// 36.7-37.5 us for 6/6/5 on the same device.
 template <class C, int P, int ST_ = 0, int HN_ = 3>
 struct PipeCfg : C {
     static constexpr int WPE =
@@ -340,8 +342,8 @@ struct PipeCfg : C {
     static constexpr int HN = HN_;
     // x-line points [0, SPLIT) are issued after the x pass, [SPLIT, SPLIT2) after the z
     // pass, [SPLIT2, N0) after the y pass.
-    static constexpr int SPLIT = ST_ == 1 ? (C::N0 + 1) / 2 : ST_ == 2 ? (C::N0 + 2) / 3 : C::N0;
-    static constexpr int SPLIT2 = ST_ == 2 ? 2 * ((C::N0 + 2) / 3) : C::N0;
+    static constexpr int SPLIT = ST_ ? (C::N0 + 1) / 2 : C::N0;
+    static constexpr int SPLIT2 = ST_ == 2 ? C::N0 - (C::N0 + 5) / 6 : C::N0;
 };
 
 // C with the streaming hints off.  The complex64 mixed-geometry launch (6^3 .. 17^3 blocks
