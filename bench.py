@@ -22,7 +22,7 @@ continuously batched server): the highest arrival rate whose p99 latency stays
 within 2 ms, for continuous batching and for the plain one-launch-per-request path, and
 the throughput ratio of the two at that cap.
 
-Prints ONE JSON line.  `python bench.py --gpus N` with N > 1 exits: this build runs on one
+Prints ONE JSON line.  `python bench.py --gpus N` with N > 1 exits: the benchmark runs on one
 GPU (batches of independent volumes shard by volume with no communication).
 """
 import argparse
@@ -388,7 +388,7 @@ def main():
                          "its measurement; for profiling runs, no JSON line")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only; batches of independent volumes "
+        print("bench.py: measures one GPU only; batches of independent volumes "
               "shard across GPUs with no communication.", file=sys.stderr)
         sys.exit(2)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))

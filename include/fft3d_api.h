@@ -5,7 +5,7 @@
  *
  * forward, unnormalised, out-of-place; element (b,x,y,z) at ((b*L + x)*L + y)*L + z,
  * interleaved complex double; `in` is not modified.  Buffers are host memory owned by the
- * caller.  This build runs the transform on the GPU: fft3d_create() allocates device
+ * caller.  This library runs the transform on the GPU: fft3d_create() allocates device
  * buffers for `batch` volumes and picks the kernel; fft3d_execute() copies in, transforms,
  * copies out, and returns with `out` complete (the driver reads it immediately,
  * bench/geom/driver.c:207-218).
