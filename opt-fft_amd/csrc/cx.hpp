@@ -46,8 +46,8 @@ FX_HD cx<T> mul_si(cx<T> a)
 // store instruction of a wave writes whole lines (the fused x pass, pencil pass B: 8^3 at
 // batch 4096 12.9 -> 11.3 us, 36^3 and 24x24x48 2-5%); where a store writes short runs
 // that later instructions complete (the pipelined y pass when a z run is shorter than a
-// 128 B line: 6^3, 12^3, 17^3) it gains nothing or loses (the complex64 mixed-geometry
-// launch 3%), so those stores stay plain.  Pencil pass B reads its intermediate with the
+// 128 B line: 6^3, 12^3, 17^3) it gains nothing or loses (17^3 with two volumes per
+// workgroup: 15% slower), so those stores stay plain.  Pencil pass B reads its intermediate with the
 // hint (the lines are read once, then overwritten: 36^3 -2.4%, 24x24x48 -7%), but pass A
 // stores it plainly: with hinted stores it leaves the Infinity Cache before pass B reads it
 // back, and 36^3 went from 69 to 77.5 us.
@@ -65,8 +65,8 @@ FX_HD void st_stream(cx<T>* p, cx<T> w)
     __builtin_nontemporal_store(V2{w.r, w.i}, reinterpret_cast<V2*>(p));
 }
 
-// H = 0: a plain access; H != 0: the streaming form above.  Both move the element as one
-// two-component vector (a struct copy can be split into two half-width stores).
+// H = 0: a plain access; H != 0: the streaming form above.  Stores move the element as one
+// two-component vector either way (a struct copy can be split into two half-width stores).
 template <int H, typename T>
 FX_HD cx<T> ld_hint(const cx<T>* p)
 {
