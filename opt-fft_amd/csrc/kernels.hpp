@@ -344,6 +344,9 @@ struct PipeCfg : C {
     // pass, [SPLIT2, N0) after the y pass.
     static constexpr int SPLIT = ST_ ? (C::N0 + 1) / 2 : C::N0;
     static constexpr int SPLIT2 = ST_ == 2 ? C::N0 - (C::N0 + 5) / 6 : C::N0;
+    static_assert(ST_ >= 0 && ST_ <= 2, "ST: 0, 1 (halves) or 2 (three parts)");
+    static_assert(0 < SPLIT && SPLIT <= SPLIT2 && SPLIT2 <= C::N0,
+                  "every x-line point is loaded exactly once");
 };
 
 // C with the streaming hints off.  The complex64 mixed-geometry launch (6^3 .. 17^3 blocks
