@@ -4,8 +4,12 @@ One step = one forward execute of the headline workload (17x17x17 complex128, ba
 inputs iid uniform(-1,1)) on device-resident buffers.  Each step uses the next of R
 (in, out) buffer pairs, R chosen so the pairs total >= 2 GiB: every working set here fits
 the MI355X's 256 MB Infinity Cache, so same-buffer repetition would time the cache, not
-HBM.  The timed region is K steps between HIP events on the launch stream, bracketed by
-device synchronisations.  The other workloads (8^3 x 4096, 36^3 x 128, 24x24x48 x 64 and
+HBM.  After W warm-up steps, exactly K steps are timed between HIP events on the launch
+stream, bracketed by device synchronisations (the contract region); then 5 windows of n
+steps each, n calibrated so a window lasts >= 20 ms, run back to back, and the reported
+rate (value, ms_per_step, roofline) is the median window's, with the spread of the windows
+under "timing" (short regions catch the clock ramp: the reference's driver insists on
+>= 20 ms samples for the same reason).  The other workloads (8^3 x 4096, 36^3 x 128, 24x24x48 x 64 and
 the 17^3 inverse) are timed the same way and reported under "secondary".
 
 "precision_variants": complex64 on the uneven batch set 6^3 x 65536, 8^3 x 32768,
@@ -88,7 +92,14 @@ def _rel_max(y, ref):
     return float(np.max(np.abs(np.asarray(y, np.complex128) - ref)) / np.max(np.abs(ref)))
 
 
-def measure(pkg, wl, steps, warmup, check=True):
+def measure(pkg, wl, steps, warmup, check=True, windows=5, window_ms=20.0):
+    """Time one workload.  After `warmup` untimed executes: (1) exactly `steps` executes
+    between HIP events on the launch stream, bracketed by device synchronisations (the
+    contract region); (2) `windows` back-to-back windows of n executes each, n calibrated
+    from (1) so that a window lasts >= window_ms (the >= 20 ms samples of the reference's
+    driver, bench/gpu/driver.cu:17-32: shorter samples catch the clock ramp).  The windows
+    follow each other on the stream with no synchronisation between them, so the device
+    stays busy through all of them.  The reported rate is the median window's."""
     f, h = pkg.fft3d, pkg.hip
     shape, batch = wl["shape"], wl["batch"]
     V = shape[0] * shape[1] * shape[2]
@@ -103,41 +114,53 @@ def measure(pkg, wl, steps, warmup, check=True):
     plan = f.Plan(shape, batch, wl["direction"], f.C128)
     stream = h.Stream()
     e0, e1 = h.Event(), h.Event()
-    for i in range(warmup):
-        plan.execute(ins[i % rot].addr, outs[i % rot].addr, stream=stream)
+    k = 0
+
+    def step():
+        nonlocal k
+        plan.execute(ins[k % rot].addr, outs[k % rot].addr, stream=stream)
+        k += 1
+
+    for _ in range(warmup):
+        step()
     h.synchronize()
     e0.record(stream)
-    for i in range(steps):
-        plan.execute(ins[i % rot].addr, outs[i % rot].addr, stream=stream)
+    for _ in range(steps):
+        step()
     e1.record(stream)
     e1.synchronize()
     h.synchronize()
-    ms = e0.elapsed_ms(e1)
-    # Per-launch durations (events bracketing each execute on its stream) for the roofline.
-    pairs = [(h.Event(), h.Event()) for _ in range(min(steps, 50))]
-    for i, (a, b) in enumerate(pairs):
-        a.record(stream)
-        plan.execute(ins[i % rot].addr, outs[i % rot].addr, stream=stream)
-        b.record(stream)
-    pairs[-1][1].synchronize()
-    per = [a.elapsed_ms(b) for a, b in pairs]
-    for a, b in pairs:
-        a.close()
-        b.close()
-    launch_ms = sum(per) / len(per)
-    res = dict(kernel=plan.kernel, rotate_pairs=rot, steps=steps, ms_total=ms,
-               ms_per_execute=ms / steps, transforms_per_s=batch * steps / (ms * 1e-3),
-               bracketed_us_mean=launch_ms * 1e3, bracketed_us_min=min(per) * 1e3)
-    # Kernel duration for the roofline: the back-to-back timed region per execute (one
-    # kernel per execute for the fused paths; the whole two-kernel sequence for pencils).
-    # Events bracketing single launches add several microseconds of event overhead.
+    contract_ms = e0.elapsed_ms(e1)
+    n = max(steps, math.ceil(window_ms / (contract_ms / steps)))
+    evs = [h.Event() for _ in range(windows + 1)]
+    evs[0].record(stream)
+    for w in range(windows):
+        for _ in range(n):
+            step()
+        evs[w + 1].record(stream)
+    evs[-1].synchronize()
+    h.synchronize()
+    last_out = (k - 1) % rot
+    wms = sorted(evs[w].elapsed_ms(evs[w + 1]) for w in range(windows))
+    for e in evs:
+        e.close()
+    med = wms[windows // 2] / n
+    res = dict(kernel=plan.kernel, rotate_pairs=rot, steps=steps,
+               contract_ms_total=contract_ms, contract_ms_per_execute=contract_ms / steps,
+               windows=windows, executes_per_window=n,
+               window_ms=[round(v, 4) for v in wms],
+               window_spread=(wms[-1] - wms[0]) / wms[windows // 2],
+               ms_per_execute=med, transforms_per_s=batch / (med * 1e-3),
+               us_per_execute_min=wms[0] / n * 1e3, us_per_execute_max=wms[-1] / n * 1e3)
+    # Kernel duration for the roofline: the median window's time per execute (one kernel per
+    # execute for the fused paths; the whole two-kernel sequence for pencils).
     algo = 2.0 * nbytes
     res["algorithmic_bytes_per_execute"] = algo
-    res["achieved_GBs"] = algo / (ms * 1e-3 / steps) / 1e9
+    res["achieved_GBs"] = algo / (med * 1e-3) / 1e9
     res["roofline_frac"] = res["achieved_GBs"] / PEAK_HBM_GBS
     if check:
-        # the last step's output pair: ((steps-1) % rot); its input is x
-        y = outs[(steps - 1) % rot].download(np.complex128, x.shape)
+        # every input buffer holds x
+        y = outs[last_out].download(np.complex128, x.shape)
         res["rel_l2_vs_numpy"] = _rel_l2(y, _np_dft(x, wl["direction"]))
     for b in ins + outs:
         b.close()
@@ -146,66 +169,99 @@ def measure(pkg, wl, steps, warmup, check=True):
     return res
 
 
-def _ref_exe(kind):
+# The reference's CPU kernels, each linked with the reference's own timing driver
+# (bench/geom/driver.c single-core, bench/mt/driver.c OpenMP; built into oracle/_ref by
+# oracle/build_ref.sh), timed at the GPU configurations' batches.  Per geometry: the
+# single-core best-exemplar candidates (winner and alternate of bench/geom/best/fft3d_best.c's
+# table) and the multi-threaded round's kernels; the faster candidate is reported, as the
+# reference's race-at-create picks it.
+CPU_CASES = [(17, 1024, "uniform"), (8, 4096, "uniform"), (36, 128, "normal_half"),
+             (6, 65536, "uniform")]
+CPU_ST = {6: ["L6_pfa", "L6_unrolled"], 8: ["L8_batchsimd", "L8_fusedaxes"],
+          17: ["L17_matrixsimd", "L17_rader"], 36: ["L36_mixedradix", "L36_pfa", "L36_pencilfused"]}
+CPU_MT = {6: ["L6_pfa"], 8: ["L8_fusedaxes", "L8_batchsimd"], 17: ["L17_winograd", "L17_matrixsimd"],
+          36: ["L36_mixedradix", "L36_pfa"]}
+
+
+def _ref_dir():
     isa = "v4" if "avx512f" in open("/proc/cpuinfo").read() else "v3"
-    p = os.path.join(ROOT, "oracle", "_ref", isa, f"ref_bench_{kind}")
-    return p if os.path.exists(p) else None
+    d = os.path.join(ROOT, "oracle", "_ref", isa)
+    return d if os.path.isdir(d) else None
 
 
-def cpu_baseline(seconds=10.0):
-    """The reference's own CPU kernels on this host, on a bounded sample of the headline
-    workload (17^3 complex128): all cores through its OpenMP kernel, plus one pinned core
-    through its best-exemplar library; the same two for 8^3 and 36^3 on 3 s samples
-    ("other_geometries").  Falls back to the oracle's numpy.fft ("port")."""
+def _ref_driver(exe, L, batch, fin, env, pin=None, samples=10):
+    """One run of the reference driver: warm-up, inner count calibrated to >= 20 ms samples,
+    `samples` timed samples (bench/geom/driver.c:166-189).  Returns its JSON record."""
     import subprocess
-    mt, st = _ref_exe("mt"), _ref_exe("st")
-    if mt and st:
-        ncores = len(os.sched_getaffinity(0))
-        env = dict(os.environ, OMP_NUM_THREADS=str(ncores), OMP_PROC_BIND="close",
+    import tempfile
+    with tempfile.NamedTemporaryFile(suffix=".json") as js:
+        cmd = ([] if pin is None else ["taskset", "-c", str(pin)]) + [
+            exe, "--L", str(L), "--batch", str(batch), "--in", fin, "--json", js.name,
+            "--samples", str(samples), "--warmup", "2", "--min-sample-ms", "20"]
+        r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=300)
+        if r.returncode != 0:
+            return None
+        return json.loads(open(js.name).read())
+
+
+def cpu_baseline():
+    """The reference's own CPU path on this host's cores through its own driver (kind
+    "reference"): all cores (OpenMP, OMP_NUM_THREADS = the cores this process may use,
+    PROC_BIND=close, PLACES=cores) and one pinned core, for every geometry of the GPU
+    configurations at their batches (6^3 at 65536 in complex128, the CPU path's only
+    precision).  value = the headline's (17^3 x 1024) all-core rate from the driver's
+    fastest sample.  Falls back to timing the oracle's numpy.fft ("port")."""
+    import tempfile
+    d = _ref_dir()
+    if d:
+        cores = sorted(os.sched_getaffinity(0))
+        env = dict(os.environ, OMP_NUM_THREADS=str(len(cores)), OMP_PROC_BIND="close",
                    OMP_PLACES="cores")
-        r = subprocess.run([mt, "--L", "17", "--batch", "1024", "--seconds", str(seconds)],
-                           capture_output=True, text=True, env=env, timeout=300)
-        r1 = subprocess.run(["taskset", "-c", "0", st, "--L", "17", "--batch", "256",
-                             "--seconds", str(seconds / 2)], capture_output=True, text=True,
-                            timeout=300)
-        # The other configurations' geometries, shorter samples: all cores and one core.
-        others = {}
-        for L, b_mt, b_st in ((8, 4096, 2048), (36, 128, 32)):
-            rm = subprocess.run([mt, "--L", str(L), "--batch", str(b_mt), "--seconds", "3"],
-                                capture_output=True, text=True, env=env, timeout=300)
-            rs = subprocess.run(["taskset", "-c", "0", st, "--L", str(L), "--batch", str(b_st),
-                                 "--seconds", "3"], capture_output=True, text=True, timeout=300)
-            if rm.returncode == 0 and rs.returncode == 0:
-                a_, b_ = (json.loads(x.stdout.strip().splitlines()[-1]) for x in (rm, rs))
-                others[f"L{L}"] = dict(
-                    all_cores=dict(value=a_["transforms_per_s"], cores=a_["threads"],
-                                   kernel=a_["kernel"], batch=b_mt, seconds=a_["seconds"]),
-                    single_core=dict(value=b_["transforms_per_s"], cores=1,
-                                     kernel=b_["kernel"], batch=b_st, seconds=b_["seconds"]))
-        if r.returncode == 0 and r1.returncode == 0:
-            m, s1 = json.loads(r.stdout.strip().splitlines()[-1]), json.loads(r1.stdout.strip().splitlines()[-1])
-            model = ""
-            try:
-                model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
-                         if l.startswith("model name")][0]
-            except Exception:
-                pass
-            return dict(value=m["transforms_per_s"], unit="transforms/s", cores=m["threads"],
+        model = ""
+        try:
+            model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo")
+                     if l.startswith("model name")][0]
+        except Exception:
+            pass
+        per = {}
+        with tempfile.TemporaryDirectory() as tmp:
+            for L, batch, dist in CPU_CASES:
+                fin = os.path.join(tmp, f"in{L}.bin")
+                make_input((L, L, L), batch, seed=2024 + batch, dist=dist).tofile(fin)
+                row = {"batch": batch}
+                for kind, table, pin, e in (("single_core", CPU_ST, cores[0], None),
+                                            ("all_cores", CPU_MT, None, env)):
+                    cands = []
+                    for k in table[L]:
+                        exe = os.path.join(d, "st" if pin is not None else "mt", k)
+                        rec = _ref_driver(exe, L, batch, fin, e, pin) if os.path.exists(exe) else None
+                        if rec:
+                            pe = rec["per_execute_seconds"]
+                            cands.append(dict(kernel=k, value=batch / pe["min"],
+                                              value_median=batch / pe["median"],
+                                              inner=rec["inner"], samples=rec["samples"]))
+                    if cands:
+                        best = max(cands, key=lambda c: c["value"])
+                        row[kind] = dict(best, cores=1 if pin is not None else len(cores),
+                                         candidates=cands)
+                os.remove(fin)
+                per[f"L{L}"] = row
+                print(f"# cpu_baseline L={L}: {row}", file=sys.stderr, flush=True)
+        head = per.get("L17", {}).get("all_cores")
+        if head:
+            return dict(value=head["value"], unit="transforms/s", cores=head["cores"],
                         kind="reference",
-                        sample=f"reference {m['kernel']} (bench/mt/impl_4, OpenMP, "
-                               f"{m['threads']} threads) on 17^3 complex128 x 1024, "
-                               f"{m['reps']} executes in {m['seconds']:.1f} s on {model}",
-                        single_core=dict(value=s1["transforms_per_s"], cores=1,
-                                         kernel=s1["kernel"],
-                                         sample=f"reference best-exemplar library "
-                                                f"(bench/geom/best, picked {s1['kernel']}) "
-                                                f"on 17^3 x 256, taskset -c 0, "
-                                                f"{s1['seconds']:.1f} s"),
-                        other_geometries=others)
+                        sample=f"reference {head['kernel']} (bench/mt/impl_4, OpenMP, "
+                               f"{head['cores']} threads) through the reference's driver "
+                               f"(bench/mt/driver.c: >=20 ms samples, fastest of "
+                               f"{head['samples']}) on 17^3 complex128 x 1024 on {model}; "
+                               f"FFTW not built (needs its configure-generated config.h)",
+                        single_core=per["L17"].get("single_core"),
+                        per_geometry=per)
     import oracle
     x = make_input((17, 17, 17), 64, seed=7, dist="uniform")
     n, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < seconds:
+    while time.perf_counter() - t0 < 10.0:
         oracle.fftn_numpy(x)
         n += x.shape[0]
     dt = time.perf_counter() - t0
@@ -254,7 +310,10 @@ def precision_mix(pkg, steps=20):
     rng = np.random.default_rng(44)
     xs = [_uniform_c64(rng, sh, b) for sh, b in C4_ITEMS]
     nbytes = sum(x.nbytes for x in xs)
-    sets = 2
+    # Input/output sets rotated so that >= 2 GiB pass between two uses of one set: the
+    # mix (882 MB in + out) and each item alone would otherwise stay in the 256 MB
+    # Infinity Cache between launches.
+    sets = max(3, math.ceil(ROTATE_BYTES / (2 * nbytes)) + 1)
     ins = [[h.DeviceBuffer(x.nbytes) for x in xs] for _ in range(sets)]
     outs = [[h.DeviceBuffer(x.nbytes) for x in xs] for _ in range(sets)]
     for s_ in range(sets):
@@ -263,16 +322,33 @@ def precision_mix(pkg, steps=20):
     stream = h.Stream()
     e0, e1 = h.Event(), h.Event()
 
-    def timed(launch):
-        for i in range(3):
-            launch(i % sets)
+    def timed(launch, nrot):
+        """Seconds per launch: the median of 3 back-to-back windows of >= 20 ms, the launch
+        argument cycling through nrot buffer sets."""
+        i = 0
+        for _ in range(3):
+            launch(i % nrot)
+            i += 1
         h.synchronize()
         e0.record(stream)
-        for i in range(steps):
-            launch(i % sets)
+        for _ in range(5):
+            launch(i % nrot)
+            i += 1
         e1.record(stream)
         e1.synchronize()
-        return e0.elapsed_ms(e1) / steps * 1e-3
+        n = max(steps, math.ceil(20.0 / (e0.elapsed_ms(e1) / 5)))
+        evs = [h.Event() for _ in range(4)]
+        evs[0].record(stream)
+        for w in range(3):
+            for _ in range(n):
+                launch(i % nrot)
+                i += 1
+            evs[w + 1].record(stream)
+        evs[-1].synchronize()
+        ws = sorted(evs[w].elapsed_ms(evs[w + 1]) for w in range(3))
+        for e in evs:
+            e.close()
+        return ws[1] / n * 1e-3
 
     def errors(bufs):
         out = {}
@@ -286,7 +362,7 @@ def precision_mix(pkg, steps=20):
            "bytes_per_launch": 2 * nbytes, "error_sample_volumes": C4_CHECK}
     mp = f.MultiPlan([(sh, b, -1) for sh, b in C4_ITEMS], f.C64)
     t = timed(lambda s_: mp.execute([b.addr for b in ins[s_]], [b.addr for b in outs[s_]],
-                                    stream=stream))
+                                    stream=stream), sets)
     h.synchronize()
     res["one_launch_fft_c64"] = dict(kernel=mp.info()[0], us=t * 1e6, GBs=2 * nbytes / t / 1e9,
                                      errors=errors(outs[0]))
@@ -294,34 +370,45 @@ def precision_mix(pkg, steps=20):
     for tag, flag in (("one_launch_mfma_bf16x3", f.MFMA_BF16), ("one_launch_mfma_fp16x3", f.MFMA_FP16)):
         mp = f.MultiPlan([(sh, b, -1) for sh, b in C4_ITEMS], f.C64, flags=flag)
         t = timed(lambda s_: mp.execute([b.addr for b in ins[s_]], [b.addr for b in outs[s_]],
-                                        stream=stream))
+                                        stream=stream), sets)
         h.synchronize()
         fl = sum(mfma_flops(sh, b) for sh, b in C4_ITEMS)
         res[tag] = dict(kernel=mp.info()[0], us=t * 1e6, GBs=2 * nbytes / t / 1e9,
                         mfma_tflops=fl / t / 1e12, mfma_util=fl / t / 1e12 / PEAK_MFMA_TFLOPS,
                         errors=errors(outs[0]))
         mp.close()
-    for tag, flag in (("fft_c64", 0), ("mfma_bf16x3", f.MFMA_BF16), ("mfma_fp16x3", f.MFMA_FP16)):
-        per = {}
-        for k, ((sh, b), x) in enumerate(zip(C4_ITEMS, xs)):
+    for lst in ins + outs:
+        for b in lst:
+            b.close()
+    # Per-geometry plans: each item with its own rotation of >= 2 GiB.
+    for tag in ("fft_c64", "mfma_bf16x3", "mfma_fp16x3"):
+        res[tag] = {}
+    for (sh, b), x in zip(C4_ITEMS, xs):
+        nrot = math.ceil(ROTATE_BYTES / (2 * x.nbytes)) + 1
+        bi = [h.DeviceBuffer(x.nbytes) for _ in range(nrot)]
+        bo = [h.DeviceBuffer(x.nbytes) for _ in range(nrot)]
+        bi[0].upload(x)
+        for q in bi[1:]:
+            h.check(h.lib().hipMemcpy(q.ptr, bi[0].ptr, x.nbytes, h.D2D), "hipMemcpy D2D")
+        ref = _np_dft(x[:C4_CHECK])
+        for tag, flag in (("fft_c64", 0), ("mfma_bf16x3", f.MFMA_BF16),
+                          ("mfma_fp16x3", f.MFMA_FP16)):
             p = f.Plan(sh, b, f.FORWARD, f.C64, flags=flag)
-            tk = timed(lambda s_: p.execute(ins[s_][k].addr, outs[s_][k].addr, stream=stream))
+            tk = timed(lambda s_: p.execute(bi[s_].addr, bo[s_].addr, stream=stream), nrot)
             h.synchronize()
-            d = dict(kernel=p.kernel, us=tk * 1e6, GBs=2 * x.nbytes / tk / 1e9)
-            y = outs[(steps - 1) % sets][k].download(np.complex64, x.shape)[:C4_CHECK]
-            ref = _np_dft(x[:C4_CHECK])
+            d = dict(kernel=p.kernel, us=tk * 1e6, GBs=2 * x.nbytes / tk / 1e9, rotate_sets=nrot)
+            y = bo[0].download(np.complex64, x.shape)[:C4_CHECK]
             d["rel_l2"] = _rel_l2(y, ref)
             d["rel_max"] = _rel_max(y, ref)
             if flag:
                 fl = mfma_flops(sh, b)
                 d["mfma_tflops"] = fl / tk / 1e12
                 d["mfma_util"] = d["mfma_tflops"] / PEAK_MFMA_TFLOPS
-            per[f"L{sh[0]}"] = d
+            res[tag][f"L{sh[0]}"] = d
             p.close()
-        res[tag] = per
-    for lst in ins + outs:
-        for b in lst:
-            b.close()
+        for q in bi + bo:
+            q.close()
+    res["rotate_sets_mix"] = sets
     return res
 
 
@@ -445,8 +532,15 @@ def main():
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": head["roofline_frac"],
                      "traffic": TRAFFIC if head["kernel"] == TRAFFIC_KERNEL else None,
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel at "
+                                       "this workload, copy-calibrated, from the committed "
+                                       "profile (not measured in this run)",
                      "kernel_us_mean": head["ms_per_execute"] * 1e3,
                      "algorithmic_bytes_per_launch": head["algorithmic_bytes_per_execute"]},
+        "timing": {k: head[k] for k in ("windows", "executes_per_window", "window_ms",
+                                        "window_spread", "us_per_execute_min",
+                                        "us_per_execute_max", "contract_ms_total",
+                                        "contract_ms_per_execute")},
         "cpu_baseline": cpu,
         "secondary": secondary,
         "precision_variants": precision,

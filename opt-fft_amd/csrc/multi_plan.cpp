@@ -1,5 +1,4 @@
 // Host side of the mixed-batch persistent launch (include/fft3d_mi355x.h, multi.hpp).
-#include <atomic>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -9,6 +8,7 @@
 #include <string>
 #include <vector>
 
+#include "claims_hip.hpp"
 #include "fft3d_mi355x.h"
 #include "multi.hpp"
 #include "multi_tasks.hpp"
@@ -25,20 +25,10 @@ struct fft3d_mx_multi {
     int2* tasks;   // device
     int ntasks;
     void* bank;    // device
-    unsigned* work;  // device: kMultiWorkSlots claim counters, one 64 B line each
-    std::atomic<unsigned> work_next;
+    unsigned* work;  // device: the claim counters `claims` hands out (claims.hpp)
+    ClaimSlots claims;
     int device;
 };
-
-// Launches of one mixed plan take successive claim counters, so up to this many may be in
-// flight at once.
-constexpr int kMultiWorkSlots = 64;
-
-static unsigned* next_work(fft3d_mx_multi* m)
-{
-    const unsigned i = m->work_next.fetch_add(1u, std::memory_order_relaxed) % kMultiWorkSlots;
-    return m->work + (size_t)i * 16;
-}
 
 template <typename T>
 static void fill_args(const fft3d_mx_multi* m, const void* const* in, void* const* out,
@@ -55,18 +45,20 @@ static void fill_args(const fft3d_mx_multi* m, const void* const* in, void* cons
     }
 }
 
+// One launch with claim counter `work` (the race passes the first counter directly; user
+// launches take theirs from m->claims).
 static hipError_t multi_launch(fft3d_mx_multi* m, const MultiDesc* k, const int2* tasks,
                                int ntasks, const void* const* in, void* const* out,
-                               hipStream_t s)
+                               unsigned* work, hipStream_t s)
 {
     if (m->prec == kC128) {
         MultiArgs<double> a;
         fill_args(m, in, out, a);
-        return k->launch(&a, tasks, ntasks, m->bank, next_work(m), s);
+        return k->launch(&a, tasks, ntasks, m->bank, work, s);
     }
     MultiArgs<float> a;
     fill_args(m, in, out, a);
-    return k->launch(&a, tasks, ntasks, m->bank, next_work(m), s);
+    return k->launch(&a, tasks, ntasks, m->bank, work, s);
 }
 
 static int upload_tasks(const std::vector<int2>& t, int2** dev)
@@ -130,12 +122,14 @@ static const MultiDesc* multi_race(fft3d_mx_multi* m, const std::vector<const Mu
                 if (upload_tasks(t, &dt) != FFT3D_MX_OK) continue;
                 bool good = true;
                 for (int w = 0; w < 2 && good; ++w)
-                    good = multi_launch(m, k, dt, (int)t.size(), ins.data(), outs.data(), s) == hipSuccess;
+                    good = multi_launch(m, k, dt, (int)t.size(), ins.data(), outs.data(), m->work,
+                                        s) == hipSuccess;
                 double kt = 1e30;
                 for (int g = 0; g < 3 && good; ++g) {
                     good = hipEventRecord(e0, s) == hipSuccess;
                     for (int r = 0; r < 3 && good; ++r)
-                        good = multi_launch(m, k, dt, (int)t.size(), ins.data(), outs.data(), s) == hipSuccess;
+                        good = multi_launch(m, k, dt, (int)t.size(), ins.data(), outs.data(),
+                                            m->work, s) == hipSuccess;
                     good = good && hipEventRecord(e1, s) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
                     float ms = 0.f;
                     if (good && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) kt = std::min(kt, (double)ms);
@@ -201,7 +195,7 @@ extern "C" fft3d_mx_multi* fft3d_mx_multi_create(const fft3d_mx_item* items, int
     bank_host_image(precision, img);
     hipError_t e = hipMalloc(&m->bank, img.size());
     if (e == hipSuccess) e = hipMemcpy(m->bank, img.data(), img.size(), hipMemcpyHostToDevice);
-    const size_t work_bytes = sizeof(unsigned) * 16 * kMultiWorkSlots;
+    const size_t work_bytes = ClaimSlots::bytes();
     if (e == hipSuccess) e = hipMalloc((void**)&m->work, work_bytes);
     if (e == hipSuccess) e = hipMemset(m->work, 0, work_bytes);
     if (e != hipSuccess) {
@@ -211,6 +205,7 @@ extern "C" fft3d_mx_multi* fft3d_mx_multi_create(const fft3d_mx_item* items, int
         delete m;
         return nullptr;
     }
+    m->claims.attach(m->work);
     const int algo = (flags & FFT3D_MX_MFMA_BF16) ? 1 : (flags & FFT3D_MX_MFMA_FP16) ? 2 : 0;
     std::vector<const MultiDesc*> cands;
     const char* forced = getenv("FFT3D_MX_MULTI_KERNEL");
@@ -256,8 +251,15 @@ extern "C" int fft3d_mx_multi_execute(fft3d_mx_multi* m, const void* const* in, 
     int cur = m->device;
     (void)hipGetDevice(&cur);
     if (cur != m->device) (void)hipSetDevice(m->device);
-    const hipError_t e = multi_launch(m, m->k, m->tasks, m->ntasks, in, out, (hipStream_t)stream);
+    const hipStream_t s = (hipStream_t)stream;
+    const hipError_t e = m->claims.run(s, [&](unsigned* w) {
+        return multi_launch(m, m->k, m->tasks, m->ntasks, in, out, w, s);
+    });
     if (cur != m->device) (void)hipSetDevice(cur);
+    if (e == HipClaimRT::busy)
+        return fail(FFT3D_MX_EINVAL, "fft3d_mx_multi_execute: %d launches of this plan were "
+                    "already captured into graphs; no claim counter is left",
+                    ClaimSlots::kSlots);
     if (e != hipSuccess)
         return fail(FFT3D_MX_EHIP, "fft3d_mx_multi_execute: %s: %s", m->k->name, hipGetErrorString(e));
     return FFT3D_MX_OK;
@@ -282,6 +284,7 @@ extern "C" int fft3d_mx_multi_get_info(const fft3d_mx_multi* m, char* kernel, si
 extern "C" void fft3d_mx_multi_destroy(fft3d_mx_multi* m)
 {
     if (!m) return;
+    m->claims.release_events();
     (void)hipFree(m->tasks);
     (void)hipFree(m->bank);
     (void)hipFree(m->work);

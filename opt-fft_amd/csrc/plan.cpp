@@ -19,12 +19,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
 #include <vector>
 
+#include "claims_hip.hpp"
 #include "cx.hpp"
 #include "fft3d_mi355x.h"
 #include "plan_internal.hpp"
@@ -138,10 +138,13 @@ static double trial(const KernelDesc& k, const char* din, char* dout, size_t wby
     return best;
 }
 
+// *best_s = 0 when no trial completed (no race buffers, or every launch failed): the
+// first candidate is returned unmeasured and the caller must not record it as wisdom.
 static const KernelDesc* race(const std::vector<const KernelDesc*>& cands, int n0, int n1,
                               int n2, int prec, long batch, const void* bank, unsigned* work,
                               double* best_s)
 {
+    *best_s = 0.0;
     const size_t vol_bytes = (size_t)n0 * n1 * n2 * elem_bytes(prec);
     // Each trial launch transforms rb volumes: ~256 MiB of input so planning stays bounded,
     // at least one volume, at most the planned batch.
@@ -202,28 +205,32 @@ static const KernelDesc* race(const std::vector<const KernelDesc*>& cands, int n
 
 using namespace fx;
 
-// Claim counters of the persistent kernels: one 64 B line per slot, slots taken round-robin
-// by successive launches, so up to kWorkSlots launches of one plan may be in flight at once
-// (on any streams) without sharing a counter.
-constexpr int kWorkSlots = 64;
-constexpr int kWorkStride = 16;  // unsigned per slot
-
 struct fft3d_mx_plan {
     int n0, n1, n2, batch, dir, prec;
     unsigned flags;
     const KernelDesc* k;
     void* bank;
-    unsigned* work;
-    std::atomic<unsigned> work_next;
+    unsigned* work;     // device: the claim counters `claims` hands out (claims.hpp)
+    ClaimSlots claims;
     double scale;
     double tune_us;
     int device;
 };
 
-static unsigned* next_work(fft3d_mx_plan* p)
+// One launch of the plan's kernel with a claim counter no running launch shares.
+static int plan_launch(fft3d_mx_plan* p, const char* who, const void* in, void* out, long batch,
+                       double scale, hipStream_t s)
 {
-    const unsigned i = p->work_next.fetch_add(1u, std::memory_order_relaxed) % kWorkSlots;
-    return p->work + (size_t)i * kWorkStride;
+    const hipError_t e = p->claims.run(s, [&](unsigned* w) {
+        return p->k->launch(in, out, batch, p->bank, scale, w, s);
+    });
+    if (e == HipClaimRT::busy)
+        return fail(FFT3D_MX_EINVAL, "%s: %d launches of this plan were already captured into "
+                    "graphs; each keeps its own claim counter and no counter is left", who,
+                    ClaimSlots::kSlots);
+    if (e != hipSuccess)
+        return fail(FFT3D_MX_EHIP, "%s: %s: %s", who, p->k->name, hipGetErrorString(e));
+    return FFT3D_MX_OK;
 }
 
 extern "C" const char* fft3d_mx_last_error(void) { return last_error(); }
@@ -325,7 +332,7 @@ static fft3d_mx_plan* create_impl(int n0, int n1, int n2, int batch, int directi
         delete p;
         return nullptr;
     }
-    const size_t work_bytes = sizeof(unsigned) * kWorkSlots * kWorkStride;
+    const size_t work_bytes = ClaimSlots::bytes();
     e = hipMalloc((void**)&p->work, work_bytes);
     if (e == hipSuccess) e = hipMemset(p->work, 0, work_bytes);
     if (e != hipSuccess) {
@@ -335,6 +342,7 @@ static fft3d_mx_plan* create_impl(int n0, int n1, int n2, int batch, int directi
         delete p;
         return nullptr;
     }
+    p->claims.attach(p->work);
 
     if (!pick) {
         if (cands.size() == 1 || (flags & FFT3D_MX_NO_TUNE) || batch == 0) {
@@ -351,7 +359,7 @@ static fft3d_mx_plan* create_impl(int n0, int n1, int n2, int batch, int directi
                 double best_s = 0.0;
                 pick = race(cands, n0, n1, n2, precision, batch, p->bank, p->work, &best_s);
                 p->tune_us = best_s * 1e6;
-                wisdom_store_locked(key, pick->name);
+                if (best_s > 0) wisdom_store_locked(key, pick->name);
             }
         }
     }
@@ -393,12 +401,10 @@ extern "C" int fft3d_mx_execute_n(fft3d_mx_plan* p, const void* in, void* out, l
     int cur = p->device;
     (void)hipGetDevice(&cur);
     if (cur != p->device) (void)hipSetDevice(p->device);
-    const hipError_t e =
-        p->k->launch(in, out, batch, p->bank, p->scale, next_work(p), (hipStream_t)stream);
+    const int rc = plan_launch(p, "fft3d_mx_execute", in, out, batch, p->scale,
+                               (hipStream_t)stream);
     if (cur != p->device) (void)hipSetDevice(cur);
-    if (e != hipSuccess)
-        return fail(FFT3D_MX_EHIP, "fft3d_mx_execute: %s: %s", p->k->name, hipGetErrorString(e));
-    return FFT3D_MX_OK;
+    return rc;
 }
 
 extern "C" int fft3d_mx_execute(fft3d_mx_plan* p, const void* in, void* out, void* stream)
@@ -426,23 +432,21 @@ extern "C" int fft3d_mx_execute_chain(fft3d_mx_plan* p, const void* in, void* ou
     // With an even m the first step writes pong, so the m-th lands in out.
     const void* src = in;
     void* dst = (m % 2) ? out : pong;
-    hipError_t e = hipSuccess;
-    for (int s = 0; s < m && e == hipSuccess; ++s) {
-        e = p->k->launch(src, dst, p->batch, p->bank, step_scale, next_work(p),
+    int rc = FFT3D_MX_OK;
+    for (int s = 0; s < m && rc == FFT3D_MX_OK; ++s) {
+        rc = plan_launch(p, "fft3d_mx_execute_chain", src, dst, p->batch, step_scale,
                          (hipStream_t)stream);
         src = dst;
         dst = dst == out ? pong : out;
     }
     if (cur != p->device) (void)hipSetDevice(cur);
-    if (e != hipSuccess)
-        return fail(FFT3D_MX_EHIP, "fft3d_mx_execute_chain: %s: %s", p->k->name,
-                    hipGetErrorString(e));
-    return FFT3D_MX_OK;
+    return rc;
 }
 
 extern "C" void fft3d_mx_destroy(fft3d_mx_plan* p)
 {
     if (!p) return;
+    p->claims.release_events();
     (void)hipFree(p->bank);
     (void)hipFree(p->work);
     delete p;

@@ -6,7 +6,6 @@ import ctypes
 import math
 import os
 import re
-import subprocess
 
 import numpy as np
 import pytest
@@ -165,18 +164,3 @@ def test_create_without_device_fails_cleanly(pkg):
     lib.fft3d_mx_clear_error()
     assert not lib.fft3d_mx_create(8, 8, 8, 4, -1, 0, 0)
     assert lib.fft3d_mx_last_error().decode() != ""
-
-
-def test_driver_unsupported_and_usage(tmp_path, pkg):
-    drv = os.path.join(ROOT, "opt-fft_amd", "bin", "fft3d_driver")
-    if not os.path.exists(drv):
-        pytest.skip("drivers not built")
-    inp = tmp_path / "in.bin"
-    oracle.make_input((7, 7, 7), 1, seed=1).tofile(inp)
-    js = tmp_path / "r.json"
-    r = subprocess.run([drv, "--L", "7", "--batch", "1", "--in", str(inp), "--json", str(js)],
-                       capture_output=True)
-    assert r.returncode == 3
-    assert '"supported":false' in js.read_text()
-    r = subprocess.run([drv, "--batch", "1"], capture_output=True)
-    assert r.returncode == 2
