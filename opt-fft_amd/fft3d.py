@@ -38,7 +38,7 @@ EXPORTED = [
     "fft3d_mx_create_named", "fft3d_mx_execute", "fft3d_mx_execute_n",
     "fft3d_mx_execute_chain", "fft3d_mx_destroy",
     "fft3d_mx_plan_get_info", "fft3d_mx_kernel_count", "fft3d_mx_kernel_name",
-    "fft3d_mx_bank_image", "fft3d_mx_volume_bytes",
+    "fft3d_mx_bank_image", "fft3d_mx_volume_bytes", "fft3d_mx_execute_noalloc",
     "fft3d_mx_multi_create", "fft3d_mx_multi_execute", "fft3d_mx_multi_get_info",
     "fft3d_mx_multi_destroy",
     "fft3d_srv_create", "fft3d_srv_submit", "fft3d_srv_wait", "fft3d_srv_run_trace",
@@ -122,6 +122,7 @@ def load():
         L.fft3d_mx_kernel_name.argtypes = [ci, ip, ip, ip, ip, ip]
         L.fft3d_mx_kernel_name.restype = ctypes.c_char_p
         L.fft3d_mx_bank_image.argtypes = [ci, vp, cs, ctypes.POINTER(cs)]
+        L.fft3d_mx_execute_noalloc.argtypes = [ci, ci, ci, cl, ci, ci, cu, vp, vp, vp]
         L.fft3d_mx_volume_bytes.argtypes = [ci, ci, ci, ci]
         L.fft3d_mx_volume_bytes.restype = cs
         L.fft3d_mx_multi_create.argtypes = [ctypes.POINTER(Item), ci, ci, cu]
@@ -259,6 +260,16 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def execute_noalloc(shape, batch, direction, precision, din, dout, stream=None, flags=0):
+    """fft3d_mx_execute_noalloc: the transform of a plan, with no plan and no allocation."""
+    s = stream.handle if isinstance(stream, hip.Stream) else stream
+    rc = load().fft3d_mx_execute_noalloc(int(shape[0]), int(shape[1]), int(shape[2]),
+                                         int(batch), int(direction), int(precision),
+                                         int(flags), din, dout, s)
+    if rc != 0:
+        raise RuntimeError(last_error())
 
 
 def _transform(x, direction, normalize):
