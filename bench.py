@@ -21,10 +21,12 @@ numpy.fft in float64, and for the matrix-core kernels the executed MFMA TFLOP/s 
 2.5 PFLOP/s dense peak.
 
 "serving": the request server under Poisson arrivals (20,000 requests, the mixed stream of
-opt-fft_amd/serving.py, 64 GB budget, 16 launches in flight, 30 us batching window for the
+opt-fft_amd/serving.py, 64 GB budget, 16 streams, 30 us batching window for the
 continuously batched server): the highest arrival rate whose p99 latency stays
-within 2 ms, for continuous batching and for the plain one-launch-per-request path, and
-the throughput ratio of the two at that cap.
+within 2 ms, for continuous batching and for the two plain one-launch-per-request paths
+(synchronous, and asynchronous with 16 launches in flight on the same streams), the
+throughput ratio of batching to each at that cap, and each mode's results checked on a
+sampled burst against numpy.fft.
 
 Prints ONE JSON line.  `python bench.py --gpus N` with N > 1 exits: the benchmark runs on one
 GPU (batches of independent volumes shard by volume with no communication).
@@ -131,7 +133,8 @@ def measure(pkg, wl, steps, warmup, check=True, windows=5, window_ms=20.0):
     e1.synchronize()
     h.synchronize()
     contract_ms = e0.elapsed_ms(e1)
-    n = max(steps, math.ceil(window_ms / (contract_ms / steps)))
+    # 25% margin: the contract region includes the clock ramp and runs slower
+    n = max(steps, math.ceil(1.25 * window_ms / (contract_ms / steps)))
     evs = [h.Event() for _ in range(windows + 1)]
     evs[0].record(stream)
     for w in range(windows):
@@ -286,23 +289,24 @@ def _uniform_c64(rng, shape, batch):
 MFMA_VPW = {(6, 6, 6): 16, (8, 8, 8): 8, (12, 12, 12): 2, (17, 17, 17): 1, (8, 8, 16): 4}
 
 
-def mfma_flops(shape, nvol):
+def mfma_flops(shape, nvol, kernel=""):
     """MFMA flops the matrix-core kernel executes for nvol volumes: per block of VPW volumes
-    and per axis pass of length N, ceil(lines/16) tiles of T x T x 3 16x16x16 products
-    (T = ceil(2N/16), three products per tile for the compensated split)."""
+    and per axis pass of length N, ceil(lines/16) tiles of TM x TK x 3 products (three per
+    tile for the compensated split), TM = ceil(2N/16); K = 16 instructions (TK = ceil(2N/16),
+    2*16*16*16 flops each), or K = 32 ones (TK = ceil(2N/32), 2*16*16*32 flops each) on the
+    axes with 2N > 16 of a "k32" kernel."""
     vpw = MFMA_VPW[tuple(shape)]
     V = shape[0] * shape[1] * shape[2]
-    per_block = 0
-    for n in shape:
-        T = (2 * n + 15) // 16
-        per_block += -(-(vpw * V // n) // 16) * T * T * 3
-    full, part = divmod(nvol, vpw)
-    insts = full * per_block
-    if part:
+
+    def per(nv):
+        fl = 0.0
         for n in shape:
-            T = (2 * n + 15) // 16
-            insts += -(-(part * V // n) // 16) * T * T * 3
-    return insts * 2.0 * 16 * 16 * 16
+            kw = 32 if (2 * n > 16 and "k32" in kernel) else 16
+            tm, tk = (2 * n + 15) // 16, (2 * n + kw - 1) // kw
+            fl += -(-(nv * V // n) // 16) * tm * tk * 3 * 2.0 * 16 * 16 * kw
+        return fl
+    full, part = divmod(nvol, vpw)
+    return full * per(vpw) + (per(part) if part else 0.0)
 
 
 def precision_mix(pkg, steps=20):
@@ -336,7 +340,7 @@ def precision_mix(pkg, steps=20):
             i += 1
         e1.record(stream)
         e1.synchronize()
-        n = max(steps, math.ceil(20.0 / (e0.elapsed_ms(e1) / 5)))
+        n = max(steps, math.ceil(25.0 / (e0.elapsed_ms(e1) / 5)))
         evs = [h.Event() for _ in range(4)]
         evs[0].record(stream)
         for w in range(3):
@@ -372,7 +376,7 @@ def precision_mix(pkg, steps=20):
         t = timed(lambda s_: mp.execute([b.addr for b in ins[s_]], [b.addr for b in outs[s_]],
                                         stream=stream), sets)
         h.synchronize()
-        fl = sum(mfma_flops(sh, b) for sh, b in C4_ITEMS)
+        fl = sum(mfma_flops(sh, b, mp.info()[0]) for sh, b in C4_ITEMS)
         res[tag] = dict(kernel=mp.info()[0], us=t * 1e6, GBs=2 * nbytes / t / 1e9,
                         mfma_tflops=fl / t / 1e12, mfma_util=fl / t / 1e12 / PEAK_MFMA_TFLOPS,
                         errors=errors(outs[0]))
@@ -401,7 +405,7 @@ def precision_mix(pkg, steps=20):
             d["rel_l2"] = _rel_l2(y, ref)
             d["rel_max"] = _rel_max(y, ref)
             if flag:
-                fl = mfma_flops(sh, b)
+                fl = mfma_flops(sh, b, p.kernel)
                 d["mfma_tflops"] = fl / tk / 1e12
                 d["mfma_util"] = d["mfma_tflops"] / PEAK_MFMA_TFLOPS
             res[tag][f"L{sh[0]}"] = d
@@ -412,20 +416,54 @@ def precision_mix(pkg, steps=20):
     return res
 
 
+def _serving_check(pkg, srv, pools, n=300, seed=21):
+    """Sampled results of one burst of the request mix through `srv`, outputs in caller
+    buffers: the first and last volume of every request against numpy.fft in float64.
+    Returns the worst relative L2 per precision."""
+    import ctypes
+    f, h, sv = pkg.fft3d, pkg.hip, pkg.serving
+    mix = sv.draw_mix(n, seed=seed)
+    rng = np.random.default_rng(seed)
+    firsts = [int(rng.integers(0, sv.MAX_BATCH + 1)) for _ in mix]
+    sizes = [b * s_[0] * s_[1] * s_[2] * (16 if p == f.C128 else 8) for s_, b, _, p in mix]
+    offs = np.concatenate([[0], np.cumsum([(z + 255) // 256 * 256 for z in sizes])])
+    arena = h.DeviceBuffer(int(offs[-1]))
+    reqs = [f.SrvRequest(*s_, b, d, p, pools.addr(s_, p, first), arena.addr + int(o))
+            for (s_, b, d, p), first, o in zip(mix, firsts, offs[:-1])]
+    srv.run_trace(reqs, np.linspace(0.0, 2e-3, n))
+    worst = {"c128": 0.0, "c64": 0.0}
+    for (s_, b, d, p), first, o in zip(mix, firsts, offs[:-1]):
+        V = s_[0] * s_[1] * s_[2]
+        dt = np.complex128 if p == f.C128 else np.complex64
+        for i in sorted({0, b - 1}):
+            y = np.empty(V, dtype=dt)
+            h.check(h.lib().hipMemcpy(y.ctypes.data, ctypes.c_void_p(arena.addr + int(o) + i * V * y.itemsize),
+                                      y.nbytes, h.D2H), "hipMemcpy D2H")
+            x = pools.host(s_, p)[first + i].reshape((1,) + s_).astype(np.complex128)
+            key = "c128" if p == f.C128 else "c64"
+            worst[key] = max(worst[key], _rel_l2(y.reshape((1,) + s_), _np_dft(x, d)))
+    arena.close()
+    return worst
+
+
 def serving(pkg, n=20000, p99_cap_ms=2.0, streams=16, window_us=30.0):
     """Load test of the request server: the highest Poisson arrival rate whose p99 latency
-    stays within p99_cap_ms, for continuous batching and for the plain path."""
+    stays within p99_cap_ms, for continuous batching and for the two plain paths (one launch
+    per request, each waited for; and the same with `streams` launches in flight), with a
+    sampled check of each mode's results."""
     f, sv = pkg.fft3d, pkg.serving
     mix = sv.draw_mix(n, seed=0)
-    pools = sv.InputPools(seed=0)
+    pools = sv.InputPools(seed=0, keep_host=True)
     reqs = sv.make_requests(mix, pools, seed=1)
     req_bytes = sum(2 * r.batch * r.n0 * r.n1 * r.n2 * (16 if r.precision == 0 else 8)
                     for r in reqs) / n
     out = {"requests": n, "mean_io_bytes_per_request": req_bytes, "p99_cap_ms": p99_cap_ms,
            "budget_bytes": 64 << 30, "streams": streams, "window_us": window_us}
-    for mode in ("plain", "batched"):
-        srv = f.Server(budget_bytes=64 << 30, streams=streams, batched=(mode == "batched"),
-                       window_us=window_us if mode == "batched" else 0.0)
+    modes = (("plain", f.SRV_PLAIN_SYNC), ("plain_async", f.SRV_PLAIN_ASYNC),
+             ("batched", f.SRV_BATCHED))
+    for mode, code in modes:
+        srv = f.Server(budget_bytes=64 << 30, streams=streams, batched=code,
+                       window_us=window_us if code == f.SRV_BATCHED else 0.0)
         ok, bad, pts = None, None, []
         rate = 4000.0
         while rate < 2e6:
@@ -448,16 +486,20 @@ def serving(pkg, n=20000, p99_cap_ms=2.0, streams=16, window_us=30.0):
                 else:
                     hi = mid
         st = srv.stats()
+        check = _serving_check(pkg, srv, pools)
         srv.close()
         out[mode] = dict(max_rate_req_s=ok["rate"] if ok else 0.0,
-                         at_max=ok, stats=st,
+                         at_max=ok, stats=st, result_check_rel_l2=check,
                          points=[{k: round(v, 4) for k, v in p.items()} for p in pts])
-        print(f"# serving {mode}: max rate {out[mode]['max_rate_req_s']:.0f}/s", file=sys.stderr,
-              flush=True)
+        print(f"# serving {mode}: max rate {out[mode]['max_rate_req_s']:.0f}/s, check {check}",
+              file=sys.stderr, flush=True)
     pools.close()
-    pl, ba = out["plain"]["at_max"], out["batched"]["at_max"]
-    if pl and ba:
-        out["throughput_ratio_at_p99_cap"] = ba["transforms_per_s"] / pl["transforms_per_s"]
+    ba = out["batched"]["at_max"]
+    for mode in ("plain", "plain_async"):
+        pl = out[mode]["at_max"]
+        if pl and ba:
+            out[f"throughput_ratio_vs_{mode}_at_p99_cap"] = (ba["transforms_per_s"] /
+                                                            pl["transforms_per_s"])
     return out
 
 

@@ -14,8 +14,9 @@
  *   * runs the other geometries (24^3, 36^3, 24x24x48) through cached plans;
  *   * keeps up to `streams` launches in flight, so requests admitted while others run
  *     overlap them on the device, and retires each launch when its event completes.
- * With batched = 0 the server is the plain path instead: one launch per request, in arrival
- * order, each waited for before the next.
+ * The plain modes serve the same requests with one plan launch each, in arrival order:
+ * FFT3D_SRV_PLAIN_SYNC waits for each launch before the next; FFT3D_SRV_PLAIN_ASYNC keeps up
+ * to `streams` of them in flight on the same streams, without packing.
  *
  * Times are seconds on the host's steady clock.  Latency of a request = completion time
  * observed by the scheduler - its arrival time. */
@@ -30,11 +31,18 @@ extern "C" {
 
 typedef struct fft3d_srv fft3d_srv;
 
+/* fft3d_srv_config.batched */
+enum {
+    FFT3D_SRV_PLAIN_SYNC = 0,  /* one launch per request, each waited for before the next */
+    FFT3D_SRV_BATCHED = 1,     /* continuous batching into mixed launches */
+    FFT3D_SRV_PLAIN_ASYNC = 2  /* one launch per request, up to `streams` in flight */
+};
+
 typedef struct fft3d_srv_config {
     size_t budget_bytes; /* admission cap on in+out bytes of unfinished requests; also the
                             size of the output arena */
     int streams;         /* launches kept in flight (1..16) */
-    int batched;         /* 1 continuous batching, 0 plain one-launch-per-request */
+    int batched;         /* FFT3D_SRV_BATCHED, FFT3D_SRV_PLAIN_SYNC or FFT3D_SRV_PLAIN_ASYNC */
     int max_items;       /* requests per mixed launch (1..16) */
     double window_us;    /* batched mode: hold a partial mixed batch open for up to this long
                             after its oldest request was admitted, so later arrivals can join

@@ -2,8 +2,10 @@
 // precisions and both directions, at several tilings each for the create-time race.
 // "pipe_pP" variants are the persistent software-pipelined form (at most P workgroups per
 // CU); "pipe_p0" is the same pass structure with one block per workgroup; "fused_" is the
-// staged form (whole block read into LDS first).
+// staged form (whole block read into LDS first); "shfl_" keeps an 8^3 volume in one
+// wavefront's registers and does the y and z axes with lane shuffles (shfl8.hpp).
 #include "kernels.hpp"
+#include "shfl8.hpp"
 
 namespace fx {
 
@@ -19,6 +21,11 @@ namespace fx {
          "pipe_" #N0 "x" #N1 "x" #N2 "_" TAG "_fwd_v" #VPW "t" #NT "p" #P);                  \
     FX_P(N0, N1, N2, T, +1, VPW, NT, PZ, P,                                                 \
          "pipe_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_v" #VPW "t" #NT "p" #P)
+
+#define FX_S(T, S, W, PERM, NAME) r.push_back(shfl8_desc<Shfl8Cfg<T, S, W, PERM>>(NAME))
+#define FX_S_BOTH(T, TAG, W, PERM, ORD)                                            \
+    FX_S(T, -1, W, PERM, "shfl_8x8x8_" TAG "_fwd_w" #W ORD);                       \
+    FX_S(T, +1, W, PERM, "shfl_8x8x8_" TAG "_bwd_w" #W ORD)
 
 void register_fused_small(std::vector<KernelDesc>& r)
 {
@@ -41,6 +48,12 @@ void register_fused_small(std::vector<KernelDesc>& r)
     FX_P_BOTH(8, 8, 8, float, "c64", 4, 256, 1, 8);
     FX_P_BOTH(8, 8, 8, float, "c64", 8, 512, 1, 4);
     FX_F_BOTH(8, 8, 8, float, "c64", 4, 256, 1);
+    // Register-resident, shuffles for y and z: W waves (volumes) per workgroup; "n" stores
+    // in natural lane order after one more shuffle, "r" from the bit-reversed lanes.
+    FX_S_BOTH(double, "c128", 4, 1, "n");
+    FX_S_BOTH(double, "c128", 4, 0, "r");
+    FX_S_BOTH(double, "c128", 8, 1, "n");
+    FX_S_BOTH(float, "c64", 4, 1, "n");
     // L = 12: 144 lines per volume per pass.
     FX_P_BOTH(12, 12, 12, double, "c128", 4, 576, 1, 2);
     FX_P_BOTH(12, 12, 12, double, "c128", 1, 192, 1, 8);

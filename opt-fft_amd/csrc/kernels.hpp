@@ -352,10 +352,12 @@ struct PipeCfg : C {
 // C with the streaming hints off.  The complex64 mixed-geometry launch (6^3 .. 17^3 blocks
 // through one persistent kernel, multi.hpp) measured 221 us with hinted x-line loads against
 // 207-209 us with plain ones on the same device.
-template <class C>
-struct PlainCfg : C {
-    static constexpr int HN = 0;
+template <class C, int HN_>
+struct HintCfg : C {
+    static constexpr int HN = HN_;
 };
+template <class C>
+using PlainCfg = HintCfg<C, 0>;
 
 // Next block of a persistent workgroup.  Each workgroup starts at block blockIdx.x and then
 // claims one ticket per iteration until it draws a block >= nblocks, so a launch with

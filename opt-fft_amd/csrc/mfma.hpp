@@ -4,17 +4,26 @@
 // 16 lines, the real product
 //       [Yr]   [ Wr  -Wi ] [Xr]          W[j][i] = exp(S 2 pi i j i / N)
 //       [Yi] = [ Wi   Wr ] [Xi]
-// of a (2N x 2N) matrix (zero-padded to a multiple of 16) with a (2N x 16) operand, on
-// v_mfma_f32_16x16x16 instructions.  Both operands are split into a high and a low
-// half-precision part (x = xh + xl, xl = x - xh) and three products xh*Wh + xl*Wh + xh*Wl
-// are accumulated in FP32 -- the compensated scheme; the dropped xl*Wl term is ~2^-32
-// (bf16) or ~2^-44 (fp16) relative.  SPLIT = 0 uses bf16 parts (fp32 range, ~16 significant
-// bits), SPLIT = 1 fp16 parts (~22 bits, range limited to |x| < 65504).  The matrix
-// fragments come from the plan-time bank and live in VGPRs for the whole kernel.
+// of a (2N x 2N) matrix (rows zero-padded to a multiple of 16, columns to a multiple of
+// the instruction's K) with a (2N x 16) operand.  K is 16 (v_mfma_f32_16x16x16*) or 32
+// (gfx950 v_mfma_f32_16x16x32_*, KW = 32, for 2N > 16: a 17-point tile is 3 x 2
+// instructions instead of 3 x 3, but its K pads 34 to 64 instead of 48, and those kernels
+// hold more operand registers; they measured 10-25% slower at N = 12 and 17 and stay in the
+// race as the "k32" variants).  Both operands are split into a high and a
+// low half-precision part (x = xh + xl, xl = x - xh, each rounded to nearest even by the
+// conversion instructions) and three products xh*Wh + xl*Wh + xh*Wl are accumulated in
+// FP32 -- the compensated scheme; the dropped xl*Wl term and the rounding of the low parts
+// are ~2^-18 of the result (bf16) or ~2^-24 (fp16).  SPLIT = 0 uses bf16 parts (~16
+// significant bits, the FP32 exponent range), SPLIT = 1 fp16 parts (~22 bits).  fp16 parts
+// hold only |x| < 65504 and the low part of a small x falls below the normal range, so with
+// SPLIT = 1 every tile of 16 lines is multiplied, before the split, by the power of two that
+// brings its largest magnitude into [0.5, 1) (a wave-wide maximum through DPP lane moves),
+// and the result by the inverse power; both scalings are exact.  The matrix fragments come
+// from the plan-time bank and live in VGPRs for the whole pass.
 //
-// Operand layouts of v_mfma_f32_16x16x16{bf16_1k,f16} (64 lanes, 4 values per lane):
-//   A: lane l holds A[l % 16][4 (l / 16) + i];  B: B[4 (l / 16) + i][l % 16];
-//   D: D[4 (l / 16) + i][l % 16],  i = 0..3.
+// Operand layouts, 64 lanes, Q = K/4 values per lane, q = 0..Q-1:
+//   A: lane l holds A[l % 16][Q (l / 16) + q];  B: B[Q (l / 16) + q][l % 16];
+//   D: D[4 (l / 16) + i][l % 16],  i = 0..3  (both K widths).
 #pragma once
 #include <stdint.h>
 
@@ -24,31 +33,22 @@ namespace fx {
 
 typedef short mf_v4s __attribute__((ext_vector_type(4)));
 typedef _Float16 mf_v4h __attribute__((ext_vector_type(4)));
+typedef __bf16 mf_v8b __attribute__((ext_vector_type(8)));
+typedef _Float16 mf_v8h __attribute__((ext_vector_type(8)));
 typedef float mf_v4f __attribute__((ext_vector_type(4)));
 
-// Round-to-nearest-even float -> bfloat16 bits (finite inputs).
-__device__ __forceinline__ short bf16_bits(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (short)(u >> 16);
-}
-__device__ __forceinline__ float bf16_value(short h)
-{
-    return __uint_as_float((uint32_t)(unsigned short)h << 16);
-}
-
-template <int SPLIT>
+// Split and product for SPLIT (0 bf16, 1 fp16) and the instruction's K width.
+template <int SPLIT, int KW>
 struct MfmaOps;
 template <>
-struct MfmaOps<0> {
+struct MfmaOps<0, 16> {
     using V = mf_v4s;
-    // Lane q of hi/lo receives the high and low parts of x.
+    static constexpr int Q = 4;
     __device__ __forceinline__ static void split(float x, V& hi, V& lo, int q)
     {
-        const short h = bf16_bits(x);
-        hi[q] = h;
-        lo[q] = bf16_bits(x - bf16_value(h));
+        const __bf16 h = (__bf16)x;
+        hi[q] = __builtin_bit_cast(short, h);
+        lo[q] = __builtin_bit_cast(short, (__bf16)(x - (float)h));
     }
     __device__ __forceinline__ static mf_v4f mma(V a, V b, mf_v4f c)
     {
@@ -56,8 +56,9 @@ struct MfmaOps<0> {
     }
 };
 template <>
-struct MfmaOps<1> {
+struct MfmaOps<1, 16> {
     using V = mf_v4h;
+    static constexpr int Q = 4;
     __device__ __forceinline__ static void split(float x, V& hi, V& lo, int q)
     {
         const _Float16 h = (_Float16)x;
@@ -69,14 +70,70 @@ struct MfmaOps<1> {
         return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
     }
 };
+template <>
+struct MfmaOps<0, 32> {
+    using V = mf_v8b;
+    static constexpr int Q = 8;
+    __device__ __forceinline__ static void split(float x, V& hi, V& lo, int q)
+    {
+        const __bf16 h = (__bf16)x;
+        hi[q] = h;
+        lo[q] = (__bf16)(x - (float)h);
+    }
+    __device__ __forceinline__ static mf_v4f mma(V a, V b, mf_v4f c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct MfmaOps<1, 32> {
+    using V = mf_v8h;
+    static constexpr int Q = 8;
+    __device__ __forceinline__ static void split(float x, V& hi, V& lo, int q)
+    {
+        const _Float16 h = (_Float16)x;
+        hi[q] = h;
+        lo[q] = (_Float16)(x - (float)h);
+    }
+    __device__ __forceinline__ static mf_v4f mma(V a, V b, mf_v4f c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
 
-// Matrix fragments of one line length N for this lane: hi and lo parts, MT x KT tiles.
-template <int N, int S, int SPLIT>
+// max(x, x moved by the DPP control CTRL) on the rows in ROWS (others keep x).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_max(float x)
+{
+    const int y = __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, x), __builtin_bit_cast(int, x),
+                                              CTRL, ROWS, 0xF, false);
+    return fmaxf(x, __builtin_bit_cast(float, y));
+}
+
+// Largest value of v over the wave, uniform: quad, half-row and row butterflies, then the
+// row broadcasts fold rows 0-2 into row 3, whose lane 63 holds the maximum.
+__device__ __forceinline__ float wave_max(float v)
+{
+    v = dpp_max<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+    v = dpp_max<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+    v = dpp_max<0x141, 0xF>(v);  // row_half_mirror
+    v = dpp_max<0x140, 0xF>(v);  // row_mirror
+    v = dpp_max<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_max<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+
+// Matrix fragments of one line length N for this lane: hi and lo parts, TM x TK tiles.
+// KW_: instruction K, 16 (also for 0) or 32 (only where 2N > 16; 16 otherwise).
+template <int N, int S, int SPLIT, int KW_ = 0>
 struct MfmaMatrix {
     static constexpr int K2 = 2 * N;
-    static constexpr int T = (K2 + 15) / 16;  // tiles along m and k
-    using Ops = MfmaOps<SPLIT>;
-    typename Ops::V hi[T][T], lo[T][T];
+    static constexpr int KW = (KW_ == 32 && K2 > 16) ? 32 : 16;   // instruction K
+    static constexpr int TM = (K2 + 15) / 16;      // tiles along m (16 rows each)
+    static constexpr int TK = (K2 + KW - 1) / KW;  // tiles along k (KW columns each)
+    using Ops = MfmaOps<SPLIT, KW>;
+    static constexpr int Q = Ops::Q;
+    typename Ops::V hi[TM][TK], lo[TM][TK];
 
     // A[m][k] of the real block matrix.
     __device__ __forceinline__ static float entry(int m, int k, const Bank<float>* __restrict__ bk)
@@ -91,12 +148,12 @@ struct MfmaMatrix {
     __device__ __forceinline__ void build(int lane, const Bank<float>* __restrict__ bk)
     {
 #pragma unroll
-        for (int mt = 0; mt < T; ++mt)
+        for (int mt = 0; mt < TM; ++mt)
 #pragma unroll
-            for (int kt = 0; kt < T; ++kt)
+            for (int kt = 0; kt < TK; ++kt)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float a = entry(16 * mt + lane % 16, 16 * kt + 4 * (lane / 16) + q, bk);
+                for (int q = 0; q < Q; ++q) {
+                    const float a = entry(16 * mt + lane % 16, KW * kt + Q * (lane / 16) + q, bk);
                     Ops::split(a, hi[mt][kt], lo[mt][kt], q);
                 }
     }
@@ -104,13 +161,13 @@ struct MfmaMatrix {
 
 // One axis pass over nlines lines (line l starts at base(l), N points STRIDE apart), in
 // place in LDS, tiles of 16 lines per wave.
-template <int N, int STRIDE, int S, int SPLIT, int NT, typename BaseFn>
+template <int N, int STRIDE, int S, int SPLIT, int NT, int KW_, typename BaseFn>
 __device__ __forceinline__ void mfma_pass(cx<float>* sm, int nlines, BaseFn base,
-                                          const MfmaMatrix<N, S, SPLIT>& A, float scl)
+                                          const MfmaMatrix<N, S, SPLIT, KW_>& A, float scl)
 {
-    using M = MfmaMatrix<N, S, SPLIT>;
-    using Ops = MfmaOps<SPLIT>;
-    constexpr int T = M::T;
+    using M = MfmaMatrix<N, S, SPLIT, KW_>;
+    using Ops = typename M::Ops;
+    constexpr int TM = M::TM, TK = M::TK, KW = M::KW, Q = M::Q;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int ntiles = (nlines + 15) / 16;
     float* smf = (float*)sm;
@@ -118,22 +175,37 @@ __device__ __forceinline__ void mfma_pass(cx<float>* sm, int nlines, BaseFn base
         const int line = tile * 16 + lane % 16;
         const bool valid = line < nlines;
         const int b = base(valid ? line : nlines - 1);
-        typename Ops::V bh[T], bl[T];
+        float v[TK][Q];
+        float mx = 0.f;
 #pragma unroll
-        for (int kt = 0; kt < T; ++kt)
+        for (int kt = 0; kt < TK; ++kt)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int k = 16 * kt + 4 * (lane / 16) + q;
-                float v = 0.f;
-                if (k < N) v = smf[2 * (b + k * STRIDE)];
-                else if (k < 2 * N) v = smf[2 * (b + (k - N) * STRIDE) + 1];
-                Ops::split(v, bh[kt], bl[kt], q);
+            for (int q = 0; q < Q; ++q) {
+                const int k = KW * kt + Q * (lane / 16) + q;
+                float x = 0.f;
+                if (k < N) x = smf[2 * (b + k * STRIDE)];
+                else if (k < 2 * N) x = smf[2 * (b + (k - N) * STRIDE) + 1];
+                v[kt][q] = x;
+                mx = fmaxf(mx, fabsf(x));
             }
+        float up = 1.f, down = scl;
+        if constexpr (SPLIT == 1) {
+            mx = wave_max(mx);
+            int e = 0;
+            if (mx > 0.f) (void)frexpf(mx, &e);
+            up = ldexpf(1.f, -e);
+            down = ldexpf(scl, e);
+        }
+        typename Ops::V bh[TK], bl[TK];
 #pragma unroll
-        for (int mt = 0; mt < T; ++mt) {
+        for (int kt = 0; kt < TK; ++kt)
+#pragma unroll
+            for (int q = 0; q < Q; ++q) Ops::split(v[kt][q] * up, bh[kt], bl[kt], q);
+#pragma unroll
+        for (int mt = 0; mt < TM; ++mt) {
             mf_v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int kt = 0; kt < T; ++kt) {
+            for (int kt = 0; kt < TK; ++kt) {
                 acc = Ops::mma(A.hi[mt][kt], bh[kt], acc);
                 acc = Ops::mma(A.hi[mt][kt], bl[kt], acc);
                 acc = Ops::mma(A.lo[mt][kt], bh[kt], acc);
@@ -142,8 +214,8 @@ __device__ __forceinline__ void mfma_pass(cx<float>* sm, int nlines, BaseFn base
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     const int m = 16 * mt + 4 * (lane / 16) + q;
-                    if (m < N) smf[2 * (b + m * STRIDE)] = acc[q] * scl;
-                    else if (m < 2 * N) smf[2 * (b + (m - N) * STRIDE) + 1] = acc[q] * scl;
+                    if (m < N) smf[2 * (b + m * STRIDE)] = acc[q] * down;
+                    else if (m < 2 * N) smf[2 * (b + (m - N) * STRIDE) + 1] = acc[q] * down;
                 }
             }
         }
@@ -157,8 +229,8 @@ struct MfmaCfg : C {
 
 // One block of VPW volumes: stage, the three axis passes, store.  CONJ: conjugate on load
 // and on store (the backward transform through forward matrices, for the mixed launch).
-// Ends with a barrier, so a caller may reuse `sm` at once.
-template <class C, int SPLIT>
+// Ends with a barrier, so a caller may reuse `sm` at once.  KW: instruction K (0 or 16, 32).
+template <class C, int SPLIT, int KW = 0>
 __device__ __forceinline__ void mfma_block(long bid, long nvol, const cx<float>* __restrict__ in,
                                            cx<float>* __restrict__ out,
                                            const Bank<float>* __restrict__ bk, float scl,
@@ -175,7 +247,7 @@ __device__ __forceinline__ void mfma_block(long bid, long nvol, const cx<float>*
         __syncthreads();
     }
     {
-        MfmaMatrix<C::N2, C::S, SPLIT> A;
+        MfmaMatrix<C::N2, C::S, SPLIT, KW> A;
         A.build(lane, bk);
         mfma_pass<C::N2, 1, C::S, SPLIT, C::NT>(
             sm, nv * C::N0 * C::N1,
@@ -188,7 +260,7 @@ __device__ __forceinline__ void mfma_block(long bid, long nvol, const cx<float>*
     }
     __syncthreads();
     {
-        MfmaMatrix<C::N1, C::S, SPLIT> A;
+        MfmaMatrix<C::N1, C::S, SPLIT, KW> A;
         A.build(lane, bk);
         mfma_pass<C::N1, C::YS, C::S, SPLIT, C::NT>(
             sm, nv * C::N0 * C::N2,
@@ -201,7 +273,7 @@ __device__ __forceinline__ void mfma_block(long bid, long nvol, const cx<float>*
     }
     __syncthreads();
     {
-        MfmaMatrix<C::N0, C::S, SPLIT> A;
+        MfmaMatrix<C::N0, C::S, SPLIT, KW> A;
         A.build(lane, bk);
         mfma_pass<C::N0, C::XS, C::S, SPLIT, C::NT>(
             sm, nv * C::PLANE,
@@ -222,14 +294,14 @@ __device__ __forceinline__ void mfma_block(long bid, long nvol, const cx<float>*
     __syncthreads();
 }
 
-template <class C, int SPLIT>
+template <class C, int SPLIT, int KW = 0>
 __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE, 8)))
     k_mfma(const cx<float>* __restrict__ in, cx<float>* __restrict__ out, long nvol,
            const Bank<float>* __restrict__ bk, float scl)
 {
     static_assert(sizeof(typename C::T) == 4, "the matrix-core variant is complex64");
     __shared__ cx<float> sm[C::LDS];
-    mfma_block<C, SPLIT>(blockIdx.x, nvol, in, out, bk, scl, false, sm);
+    mfma_block<C, SPLIT, KW>(blockIdx.x, nvol, in, out, bk, scl, false, sm);
 }
 
 }  // namespace fx

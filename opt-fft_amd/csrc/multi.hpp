@@ -40,6 +40,7 @@ struct MultiPreset;
 template <typename T>
 struct MultiPreset<T, 0> {
     static constexpr int NT = 320;
+    static constexpr int HN = 0;  // streaming hints (FusedCfg::HN): none
     using G0 = FusedCfg<6, 6, 6, T, -1, 8, NT, 1, 0, 1>;
     using G1 = FusedCfg<8, 8, 8, T, -1, 5, NT, 1, 0, 1>;
     using G2 = FusedCfg<12, 12, 12, T, -1, 2, NT, 1, 0, 1>;
@@ -50,11 +51,35 @@ struct MultiPreset<T, 0> {
 template <typename T>
 struct MultiPreset<T, 1> {
     static constexpr int NT = 640;
+    static constexpr int HN = 0;
     using G0 = FusedCfg<6, 6, 6, T, -1, 16, NT, 1, 0, 1>;
     using G1 = FusedCfg<8, 8, 8, T, -1, 10, NT, 1, 0, 1>;
     using G2 = FusedCfg<12, 12, 12, T, -1, 4, NT, 1, 0, 1>;
     using G3 = FusedCfg<17, 17, 17, T, -1, 2, NT, 0, 0, 1>;
     using G4 = FusedCfg<8, 8, 16, T, -1, 5, NT, 1, 0, 1>;
+};
+
+// 640 threads, every geometry's block sized to the LDS the two 17^3 volumes need (78.6 KB
+// at complex64): the small geometries move 2-4x the bytes per task of preset 1, in the same
+// two resident workgroups per CU.
+template <typename T>
+struct MultiPreset<T, 2> {
+    static constexpr int NT = 640;
+    static constexpr int HN = 0;
+    using G0 = FusedCfg<6, 6, 6, T, -1, 38, NT, 1, 0, 1>;
+    using G1 = FusedCfg<8, 8, 8, T, -1, 17, NT, 1, 0, 1>;
+    using G2 = FusedCfg<12, 12, 12, T, -1, 5, NT, 1, 0, 1>;
+    using G3 = FusedCfg<17, 17, 17, T, -1, 2, NT, 0, 0, 1>;
+    using G4 = FusedCfg<8, 8, 16, T, -1, 9, NT, 1, 0, 1>;
+};
+// Preset 2 with nontemporal input loads (3) or loads and stores (4).
+template <typename T>
+struct MultiPreset<T, 3> : MultiPreset<T, 2> {
+    static constexpr int HN = 1;
+};
+template <typename T>
+struct MultiPreset<T, 4> : MultiPreset<T, 2> {
+    static constexpr int HN = 3;
 };
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
@@ -75,11 +100,11 @@ struct MultiCfg {
 
 // Phase `ph` (0: x from HBM, 1: z, 2: y to HBM) of one block of one item.  Phases are
 // separated by workgroup barriers; no register state crosses a barrier.
-template <class C>
+template <class C, int HN>
 FX_HD void multi_phase(int ph, int tid, long b, const MultiItem<typename C::T>& it,
                        cx<typename C::T>* sm, const Bank<typename C::T>* __restrict__ bk)
 {
-    using K = FusedPipe<PlainCfg<C>>;
+    using K = FusedPipe<HintCfg<C, HN>>;
     if (ph == 0) {
         typename K::Regs R;
         K::load_x(tid, b, it.nvol, it.in, R);
@@ -104,11 +129,11 @@ FX_HD void multi_task_phase(int ph, int tid, int2 task, const MultiArgs<T>& a, c
     using P = MultiPreset<T, PRESET>;
     const MultiItem<T>& it = a.item[task.x];
     switch (it.code) {
-    case 0: multi_phase<typename P::G0>(ph, tid, task.y, it, sm, bk); break;
-    case 1: multi_phase<typename P::G1>(ph, tid, task.y, it, sm, bk); break;
-    case 2: multi_phase<typename P::G2>(ph, tid, task.y, it, sm, bk); break;
-    case 3: multi_phase<typename P::G3>(ph, tid, task.y, it, sm, bk); break;
-    default: multi_phase<typename P::G4>(ph, tid, task.y, it, sm, bk); break;
+    case 0: multi_phase<typename P::G0, P::HN>(ph, tid, task.y, it, sm, bk); break;
+    case 1: multi_phase<typename P::G1, P::HN>(ph, tid, task.y, it, sm, bk); break;
+    case 2: multi_phase<typename P::G2, P::HN>(ph, tid, task.y, it, sm, bk); break;
+    case 3: multi_phase<typename P::G3, P::HN>(ph, tid, task.y, it, sm, bk); break;
+    default: multi_phase<typename P::G4, P::HN>(ph, tid, task.y, it, sm, bk); break;
     }
 }
 

@@ -331,10 +331,12 @@ static void dispatch(fft3d_srv* s)
         const int stream = free_stream(s);
         if (stream < 0) return;
         Req* head = s->ready.front();
-        if (!s->cfg.batched || head->code < 0) {
+        if (s->cfg.batched != FFT3D_SRV_BATCHED || head->code < 0) {
+            // plain modes, and the plan-path geometries of the batched mode
+            const bool sync = s->cfg.batched == FFT3D_SRV_PLAIN_SYNC;
             s->ready.pop_front();
-            if (!launch_plan(s, head, stream, !s->cfg.batched)) fail_reqs(s, {head});
-            if (!s->cfg.batched) retire(s);
+            if (!launch_plan(s, head, stream, sync)) fail_reqs(s, {head});
+            if (sync) retire(s);
             continue;
         }
         // A partial mixed batch stays open for up to window_us after its oldest request was
@@ -462,7 +464,8 @@ extern "C" fft3d_srv* fft3d_srv_create(const fft3d_srv_config* cfg)
 {
     if (!cfg || cfg->budget_bytes < (1u << 20) || cfg->streams < 1 || cfg->streams > kMaxStreams ||
         cfg->max_items < 1 || cfg->max_items > kMultiMaxItems || !(cfg->window_us >= 0) ||
-        cfg->window_us > 1e6) {
+        cfg->window_us > 1e6 || cfg->batched < FFT3D_SRV_PLAIN_SYNC ||
+        cfg->batched > FFT3D_SRV_PLAIN_ASYNC) {
         fail(FFT3D_MX_EINVAL, "fft3d_srv_create: bad configuration");
         return nullptr;
     }
@@ -486,14 +489,15 @@ extern "C" fft3d_srv* fft3d_srv_create(const fft3d_srv_config* cfg)
             if (d.prec == prec && d.algo == 0 && d.launch_dev && !s->mk[prec]) s->mk[prec] = &d;
         ok = ok && s->mk[prec] && make_slots(s, prec, cfg->streams);
     }
-    // Plans for every geometry, both precisions and directions (the plain path uses them
-    // for all requests, the batched path for the pencil geometries).
+    // Plans for every geometry, both precisions and directions (the plain modes use them
+    // for all requests, the batched mode for the pencil geometries).
     static const int geo[][3] = {{6, 6, 6}, {8, 8, 8}, {12, 12, 12}, {17, 17, 17}, {8, 8, 16},
                                  {24, 24, 24}, {36, 36, 36}, {24, 24, 48}};
     for (const auto& g : geo)
         for (int prec = 0; prec < 2 && ok; ++prec)
             for (int dir = -1; dir <= 1 && ok; dir += 2) {
-                if (cfg->batched && multi_code(g[0], g[1], g[2]) >= 0) continue;
+                if (cfg->batched == FFT3D_SRV_BATCHED && multi_code(g[0], g[1], g[2]) >= 0)
+                    continue;
                 fft3d_mx_plan* p = fft3d_mx_create(g[0], g[1], g[2], kMaxBatch, dir, prec, 0);
                 ok = p != nullptr;
                 if (ok) s->plans[plan_key(g[0], g[1], g[2], prec, dir)] = p;

@@ -385,13 +385,19 @@ class MultiPlan:
             pass
 
 
+SRV_PLAIN_SYNC, SRV_BATCHED, SRV_PLAIN_ASYNC = 0, 1, 2
+
+
 class Server:
-    """fft3d_srv: continuous batching (batched=True) or the plain one-launch-per-request
-    path (batched=False)."""
+    """fft3d_srv: continuous batching (batched=True or SRV_BATCHED), the plain
+    one-launch-per-request path waited for request by request (batched=False or
+    SRV_PLAIN_SYNC), or the same with `streams` launches in flight (SRV_PLAIN_ASYNC)."""
 
     def __init__(self, budget_bytes=64 << 30, streams=8, batched=True, max_items=16,
                  window_us=0.0):
-        cfg = SrvConfig(int(budget_bytes), int(streams), 1 if batched else 0, int(max_items),
+        mode = SRV_BATCHED if batched is True else SRV_PLAIN_SYNC if batched is False \
+            else int(batched)
+        cfg = SrvConfig(int(budget_bytes), int(streams), mode, int(max_items),
                         float(window_us))
         h = load().fft3d_srv_create(ctypes.byref(cfg))
         if not h:

@@ -30,11 +30,13 @@ def draw_mix(n, seed=0):
 
 class InputPools:
     """Device-resident uniform(-1,1) inputs: per geometry and precision, 2*MAX_BATCH
-    volumes, so a request of batch b can read any window of b consecutive volumes."""
+    volumes, so a request of batch b can read any window of b consecutive volumes.
+    keep_host=True keeps the host copies (host(shape, prec)) for checking results."""
 
-    def __init__(self, seed=0):
+    def __init__(self, seed=0, keep_host=False):
         rng = np.random.default_rng(seed)
         self.bufs = {}
+        self.hosts = {}
         for shape in GEOMS:
             V = shape[0] * shape[1] * shape[2]
             for prec in (fft3d.C128, fft3d.C64):
@@ -44,6 +46,12 @@ class InputPools:
                 buf = hip.DeviceBuffer(x.nbytes)
                 buf.upload(x)
                 self.bufs[(shape, prec)] = (buf, x.itemsize * V)
+                if keep_host:
+                    self.hosts[(shape, prec)] = x
+
+    def host(self, shape, prec):
+        """Host copy of a pool, (2*MAX_BATCH, V) (needs keep_host=True)."""
+        return self.hosts[(tuple(shape), prec)]
 
     def addr(self, shape, prec, first):
         buf, vb = self.bufs[(shape, prec)]

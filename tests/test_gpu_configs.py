@@ -1,0 +1,174 @@
+"""GPU parity at the configurations' own sizes, beyond the headline shapes:
+
+* every persistent kernel variant (forced by name) at a batch of at least four times its
+  largest resident grid, so each workgroup walks several claimed blocks with the next
+  block's lines prefetched, against numpy.fft on the whole batch;
+* the uneven complex64 mix in ONE persistent launch (6^3 x 65536, 8^3 x 32768,
+  12^3 x 8192, 17^3 x 2048) as the FFT and as both matrix-core splits: 256 sampled volumes
+  per item against numpy.fft in float64, Parseval and linearity over the full batch;
+* a stream of 2,000 requests of the serving mix (batches log-uniform in [1, 512], both
+  directions, complex128 and complex64) through the continuously batched server under a
+  64 GB budget, outputs in caller buffers: every request checked on sampled volumes, and a
+  sample of requests bit-identical when served alone.
+
+Tolerances (relative L2): complex128 1e-13; complex64 2e-6; matrix-core bf16x3 2e-5,
+fp16x3 6e-7."""
+import ctypes
+import math
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+CUS = 256
+TOL = {0: 1e-13, 1: 2e-6}
+
+
+def _persistent_groups():
+    """(shape, prec, direction) -> [(name, batch needed)] for every persistent variant."""
+    import __graft_entry__
+    f = __graft_entry__.load_package().fft3d
+    groups = {}
+    for name, shape, prec, d in f.kernels():
+        m = re.search(r"_v(\d+)t\d+p(\d+)", name)
+        if m and int(m.group(2)) > 0:
+            vpw, P = int(m.group(1)), int(m.group(2))
+            need = 4 * P * CUS * vpw
+        else:
+            m = re.search(r"_s(\d+)t\d+b\d+c\d+p(\d+)", name)
+            if not (m and int(m.group(2)) > 0):
+                continue
+            slabs, P = int(m.group(1)), int(m.group(2))
+            need = math.ceil(4 * P * CUS / math.ceil(shape[0] / slabs))
+        groups.setdefault((shape, prec, d), []).append((name, need))
+    return groups
+
+
+GROUPS = _persistent_groups()
+
+
+@pytest.mark.parametrize("key", sorted(GROUPS), ids=lambda k: "%dx%dx%d_%s_%s" % (
+    *k[0], "c128" if k[1] == 0 else "c64", "fwd" if k[2] < 0 else "bwd"))
+def test_persistent_variants_walk_many_blocks(gpu, key):
+    f = gpu.fft3d
+    shape, prec, d = key
+    batch = max(n for _, n in GROUPS[key])
+    dt = np.complex128 if prec == 0 else np.complex64
+    x = oracle.make_input(shape, batch, seed=31 + batch, dtype=dt)
+    want = oracle.fftn_numpy(x, sign=d)
+    for name, _ in GROUPS[key]:
+        print(f"{name} B={batch} ...", end=" ", flush=True)
+        plan = f.Plan(shape, batch, d, prec, kernel=name)
+        y = plan.run(x)
+        plan.close()
+        err = oracle.rel_l2(y, want)
+        print(f"rel_l2={err:.2e}", flush=True)
+        assert err <= TOL[prec], (name, err)
+
+
+# --------------------------------------------------------------------------- uneven mix
+C4 = [((6, 6, 6), 65536), ((8, 8, 8), 32768), ((12, 12, 12), 8192), ((17, 17, 17), 2048)]
+C4_TOL = {0: 2e-6, 4: 2e-5, 8: 6e-7}   # flags -> rel L2 bar
+
+
+def _c64(seed, shape, batch):
+    return oracle.make_input(shape, batch, seed=seed, dtype=np.complex64)
+
+
+def _energy(a):
+    a = a.reshape(a.shape[0], -1)
+    return np.sum(a.real.astype(np.float64) ** 2 + a.imag.astype(np.float64) ** 2)
+
+
+@pytest.mark.parametrize("flags", [0, 4, 8], ids=["fft", "mfma_bf16x3", "mfma_fp16x3"])
+def test_uneven_mix_one_launch_full_size(gpu, flags):
+    f = gpu.fft3d
+    tol = C4_TOL[flags]
+    x1 = [_c64(100 + k, s, b) for k, (s, b) in enumerate(C4)]
+    x2 = [_c64(200 + k, s, b) for k, (s, b) in enumerate(C4)]
+    a, c = 0.75 - 0.5j, -1.25 + 0.25j
+    x3 = [(a * p + c * q).astype(np.complex64) for p, q in zip(x1, x2)]
+    m = f.MultiPlan([(s, b, -1) for s, b in C4], precision=f.C64, flags=flags)
+    name, ntasks, grid = m.info()
+    print(f"{name}: {ntasks} tasks on {grid} workgroups", flush=True)
+    y1, y2, y3 = m.run(x1), m.run(x2), m.run(x3)
+    m.close()
+    rng = np.random.default_rng(5)
+    for k, (s, b) in enumerate(C4):
+        V = s[0] * s[1] * s[2]
+        idx = np.sort(rng.choice(b, 256, replace=False))
+        ref = oracle.fftn_numpy(x1[k][idx].astype(np.complex128))
+        err = oracle.rel_l2(y1[k][idx], ref)
+        pars = abs(_energy(y1[k]) - V * _energy(x1[k])) / (V * _energy(x1[k]))
+        lin = oracle.rel_l2(y3[k], a * y1[k].astype(np.complex128) + c * y2[k].astype(np.complex128))
+        print(f"  {s} x {b}: rel_l2 {err:.2e}  parseval {pars:.2e}  linearity {lin:.2e}",
+              flush=True)
+        assert np.all(np.isfinite(y1[k])) and np.all(np.isfinite(y3[k]))
+        assert err <= tol, (s, err)
+        assert pars <= 2 * tol, (s, pars)
+        assert lin <= 3 * tol, (s, lin)
+
+
+# ------------------------------------------------------------------------ serving stream
+def _download_volumes(buf_addr, dtype, V, first, count):
+    import __graft_entry__
+    h = __graft_entry__.load_package().hip
+    out = np.empty((count, V), dtype=dtype)
+    h.check(h.lib().hipMemcpy(out.ctypes.data, ctypes.c_void_p(buf_addr + first * V * out.itemsize),
+                              out.nbytes, h.D2H), "hipMemcpy D2H")
+    return out
+
+
+def test_serving_stream_2000_requests_full_batches(gpu):
+    f, h, sv = gpu.fft3d, gpu.hip, gpu.serving
+    n = 2000
+    mix = sv.draw_mix(n, seed=11)
+    assert max(b for _, b, _, _ in mix) > 256
+    pools = sv.InputPools(seed=4, keep_host=True)
+    rng = np.random.default_rng(12)
+    firsts = [int(rng.integers(0, sv.MAX_BATCH + 1)) for _ in mix]
+    sizes = [b * s[0] * s[1] * s[2] * (16 if p == f.C128 else 8) for s, b, _, p in mix]
+    offs = np.concatenate([[0], np.cumsum([(z + 255) // 256 * 256 for z in sizes])])
+    arena = h.DeviceBuffer(int(offs[-1]))
+    reqs = [f.SrvRequest(*s, b, d, p, pools.addr(s, p, first), arena.addr + int(o))
+            for (s, b, d, p), first, o in zip(mix, firsts, offs[:-1])]
+    srv = f.Server(budget_bytes=64 << 30, streams=16, batched=True, window_us=30.0)
+    arrivals = sv.poisson_arrivals(n, 60000.0, seed=13)
+    lat = srv.run_trace(reqs, arrivals)
+    st = srv.stats()
+    print(st, f"p50 {np.percentile(lat, 50) * 1e3:.3f} ms p99 {np.percentile(lat, 99) * 1e3:.3f} ms",
+          flush=True)
+    assert st["completed"] == n and np.all(lat > 0)
+    assert st["mixed_launches"] > 0 and st["peak_inflight_bytes"] <= 64 << 30
+    worst = {0: 0.0, 1: 0.0}
+    samples = {}
+    for k, ((s, b, d, p), first, o) in enumerate(zip(mix, firsts, offs[:-1])):
+        V = s[0] * s[1] * s[2]
+        dt = np.complex128 if p == f.C128 else np.complex64
+        pick = sorted({0, b // 3, (2 * b) // 3, b - 1})
+        host = pools.host(s, p)
+        for i in pick:
+            y = _download_volumes(arena.addr + int(o), dt, V, i, 1).reshape((1,) + s)
+            x = host[first + i].reshape((1,) + s).astype(np.complex128)
+            err = oracle.rel_l2(y, oracle.fftn_numpy(x, sign=d))
+            worst[p] = max(worst[p], err)
+            assert err <= TOL[p], (k, s, b, d, p, i, err)
+        if k % 20 == 0:
+            samples[k] = _download_volumes(arena.addr + int(o), dt, V, 0, b)
+    print(f"worst rel_l2: c128 {worst[0]:.2e}  c64 {worst[1]:.2e}", flush=True)
+    # the same requests alone, one at a time, into a scratch output
+    for k, y_mix in samples.items():
+        s, b, d, p = mix[k]
+        scratch = h.DeviceBuffer(sizes[k])
+        rid = srv.submit(s, b, d, p, pools.addr(s, p, firsts[k]), scratch.addr)
+        srv.wait(rid)
+        alone = _download_volumes(scratch.addr, y_mix.dtype, y_mix.shape[1], 0, b)
+        scratch.close()
+        assert np.array_equal(alone.view(np.uint8), y_mix.view(np.uint8)), (k, s, b)
+    srv.close()
+    arena.close()
+    pools.close()

@@ -1,9 +1,10 @@
 """Matrix-core DFT-as-matrix variant (complex64 only, opt-in by plan flag).
 
 Each axis pass is a real (2N x 2N) x (2N x 16) product per 16 lines on
-v_mfma_f32_16x16x16 instructions, both operands split into high + low half-precision parts,
-hh + hl + lh accumulated in FP32.  Checked against the exact oracle; the bars below are the
-variants' stated accuracy, looser than the FFT path's 2e-6."""
+v_mfma_f32_16x16x{16,32} instructions, both operands split into high + low half-precision
+parts, hh + hl + lh accumulated in FP32, every tile scaled by a power of two before an fp16
+split.  Checked against the exact oracle; the bars below are the variants' stated accuracy,
+looser than the FFT path's 2e-6, and hold at any input scale."""
 import numpy as np
 import pytest
 
@@ -33,6 +34,27 @@ def test_mfma_plans_match_oracle(gpu, split):
                 print(f"{name} dir={direction} B={batch} rel_l2={err:.2e}", flush=True)
                 assert np.all(np.isfinite(y))
                 assert err <= BARS[split], (shape, direction, batch, err)
+
+
+@pytest.mark.parametrize("split", ["bf16", "fp16"])
+@pytest.mark.parametrize("scale,dc", [(1e-3, 0.0), (1e-6, 0.0), (300.0, 0.0), (1e4, 0.0),
+                                      (1.0, 300.0), (1e-30, 0.0)])
+def test_mfma_accuracy_at_any_input_scale(gpu, split, scale, dc):
+    """Small, large and DC-heavy inputs: finite results at the variant's stated bar (fp16
+    parts hold only |x| < 65504 and lose bits below 6e-5 without the per-line scaling)."""
+    f = gpu.fft3d
+    flag = f.MFMA_BF16 if split == "bf16" else f.MFMA_FP16
+    for shape in [(17, 17, 17), (8, 8, 8), (12, 12, 12)]:
+        x = (oracle.make_input(shape, 5, seed=19, dtype=np.complex128) * scale + dc)
+        x = x.astype(np.complex64)
+        plan = f.Plan(shape, 5, f.FORWARD, f.C64, flags=flag)
+        name = plan.kernel
+        y = plan.run(x)
+        plan.close()
+        err = oracle.rel_l2(y, oracle.fftn_numpy(x.astype(np.complex128)))
+        print(f"{name} scale={scale} dc={dc} rel_l2={err:.2e}", flush=True)
+        assert np.all(np.isfinite(y))
+        assert err <= BARS[split], (shape, scale, dc, err)
 
 
 def test_mfma_normalized_round_trip(gpu):

@@ -68,11 +68,14 @@ def test_mix_matches_oracle_and_alone(gpu, streams, window_us):
         b.close()
 
 
-def test_plain_path_matches_oracle(gpu):
+@pytest.mark.parametrize("mode,streams", [(0, 1), (2, 4), (2, 16)])
+def test_plain_path_matches_oracle(gpu, mode, streams):
+    """The plain one-launch-per-request modes: synchronous, and asynchronous with several
+    launches in flight."""
     f = gpu.fft3d
-    mix = _mix(gpu, 30, 16, seed=8)
+    mix = _mix(gpu, 60, 16, seed=8)
     xs, ins, outs = _setup(gpu, mix, 300)
-    srv = f.Server(budget_bytes=1 << 30, streams=1, batched=False)
+    srv = f.Server(budget_bytes=1 << 30, streams=streams, batched=mode)
     reqs = [f.SrvRequest(*shape, b, d, p, bi.addr, bo.addr)
             for (shape, b, d, p), bi, bo in zip(mix, ins, outs)]
     srv.run_trace(reqs, np.linspace(0, 1e-3, len(reqs)))
@@ -86,7 +89,8 @@ def test_plain_path_matches_oracle(gpu):
 def test_bad_configuration_is_rejected(gpu):
     f = gpu.fft3d
     for kw in (dict(streams=0), dict(streams=17), dict(max_items=0), dict(max_items=17),
-               dict(window_us=-1.0), dict(window_us=2e6), dict(budget_bytes=1 << 10)):
+               dict(window_us=-1.0), dict(window_us=2e6), dict(budget_bytes=1 << 10),
+               dict(batched=3), dict(batched=-1)):
         with pytest.raises(RuntimeError):
             f.Server(**kw)
 
