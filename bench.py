@@ -508,6 +508,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--windows", type=int, default=5,
+                    help="timed windows after the contract region (the value is their median)")
+    ap.add_argument("--window-ms", type=float, default=20.0, help="minimum window length")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -537,14 +540,15 @@ def main():
         if not wl:
             print(f"bench.py: unknown workload {a.workload}", file=sys.stderr)
             sys.exit(2)
-        print(f"# {a.workload}: {measure(pkg, wl[0], a.steps, a.warmup)}", flush=True)
+        print(f"# {a.workload}: {measure(pkg, wl[0], a.steps, a.warmup, windows=a.windows, window_ms=a.window_ms)}", flush=True)
         return
-    head = measure(pkg, HEADLINE, a.steps, a.warmup)
+    head = measure(pkg, HEADLINE, a.steps, a.warmup, windows=a.windows, window_ms=a.window_ms)
     print(f"# headline {HEADLINE['name']}: {head}", file=sys.stderr, flush=True)
     secondary = {}
     if not a.no_secondary:
         for wl in SECONDARY:
-            secondary[wl["name"]] = measure(pkg, wl, max(10, a.steps // 2), a.warmup)
+            secondary[wl["name"]] = measure(pkg, wl, max(10, a.steps // 2), a.warmup,
+                                            windows=a.windows, window_ms=a.window_ms)
             print(f"# {wl['name']}: {secondary[wl['name']]}", file=sys.stderr, flush=True)
     precision = None if a.no_precision else precision_mix(pkg)
     if precision:

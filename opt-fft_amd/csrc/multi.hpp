@@ -72,15 +72,8 @@ struct MultiPreset<T, 2> {
     using G3 = FusedCfg<17, 17, 17, T, -1, 2, NT, 0, 0, 1>;
     using G4 = FusedCfg<8, 8, 16, T, -1, 9, NT, 1, 0, 1>;
 };
-// Preset 2 with nontemporal input loads (3) or loads and stores (4).
-template <typename T>
-struct MultiPreset<T, 3> : MultiPreset<T, 2> {
-    static constexpr int HN = 1;
-};
-template <typename T>
-struct MultiPreset<T, 4> : MultiPreset<T, 2> {
-    static constexpr int HN = 3;
-};
+// (Preset 2 with nontemporal loads, or loads and stores, took 207.8 us on the complex64
+// C4 mix against 186.8 us plain.)
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 

@@ -13,6 +13,7 @@ const std::vector<KernelDesc>& registry()
         register_pencil(v);
         register_mfma(v);
         register_f17(v);
+        register_dense(v);
         return v;
     }();
     return r;

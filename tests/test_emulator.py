@@ -32,7 +32,7 @@ def emu():
 
 
 def parse(name):
-    m = re.search(r"_(\d+)x(\d+)x(\d+)_(c128|c64)_(fwd|bwd)_[svw](\d+)", name)
+    m = re.search(r"_(\d+)x(\d+)x(\d+)_(c128|c64)_(fwd|bwd)_[svwl](\d+)", name)
     shape = tuple(int(m.group(k)) for k in (1, 2, 3))
     return shape, (0 if m.group(4) == "c128" else 1), (-1 if m.group(5) == "fwd" else 1), int(m.group(6))
 

@@ -172,3 +172,23 @@ def test_serving_stream_2000_requests_full_batches(gpu):
     srv.close()
     arena.close()
     pools.close()
+
+
+# --------------------------------------------------------------------------- dense floor
+@pytest.mark.parametrize("shape,batch", [((17, 17, 17), 64), ((36, 36, 36), 8), ((24, 24, 48), 6),
+                                         ((8, 8, 8), 512)])
+def test_dense_floor_agrees_with_the_fast_kernels(gpu, shape, batch):
+    """The dense row-column floor (no factorization) and the plan's fast kernel, both on the
+    device, against each other and the oracle."""
+    f = gpu.fft3d
+    x = oracle.make_input(shape, batch, seed=3 + batch, dist="normal_half")
+    name = "dense_%dx%dx%d_c128_fwd_l1" % shape
+    dense = f.Plan(shape, batch, f.FORWARD, f.C128, kernel=name)
+    fast = f.Plan(shape, batch, f.FORWARD, f.C128)
+    yd, yf = dense.run(x), fast.run(x)
+    print(f"{name} vs {fast.kernel}", flush=True)
+    dense.close()
+    fast.close()
+    want = oracle.fftn_numpy(x)
+    assert oracle.rel_l2(yd, want) <= 1e-13
+    assert oracle.rel_l2(yd, yf) <= 1e-13
