@@ -215,15 +215,31 @@ struct fft3d_mx_plan {
     double scale;
     double tune_us;
     int device;
+    int cus;            // compute units of the device
 };
+
+// True when a launch of `batch` volumes may draw blocks from a claim counter.  A persistent
+// launch claims only when its blocks number at least twice its grid (claim_counter), and its
+// grid is at least min(CUs, blocks), so fewer than 2 x CUs blocks never claim: such launches
+// skip the counter bookkeeping (and, on several streams, its event).
+static bool may_claim(const fft3d_mx_plan* p, long batch)
+{
+    if (p->k->persist <= 0) return false;
+    const long units = p->k->kind == kPencil ? batch * p->n0 : batch;
+    const long nblocks = (units + p->k->vpw - 1) / p->k->vpw;
+    return nblocks >= 2L * p->cus;
+}
 
 // One launch of the plan's kernel with a claim counter no running launch shares.
 static int plan_launch(fft3d_mx_plan* p, const char* who, const void* in, void* out, long batch,
                        double scale, hipStream_t s)
 {
-    const hipError_t e = p->claims.run(s, [&](unsigned* w) {
-        return p->k->launch(in, out, batch, p->bank, scale, w, s);
-    });
+    const hipError_t e =
+        may_claim(p, batch)
+            ? p->claims.run(s, [&](unsigned* w) {
+                  return p->k->launch(in, out, batch, p->bank, scale, w, s);
+              })
+            : p->k->launch(in, out, batch, p->bank, scale, nullptr, s);
     if (e == HipClaimRT::busy)
         return fail(FFT3D_MX_EINVAL, "%s: %d launches of this plan were already captured into "
                     "graphs; each keeps its own claim counter and no counter is left", who,
@@ -316,6 +332,9 @@ static fft3d_mx_plan* create_impl(int n0, int n1, int n2, int batch, int directi
     p->scale = (flags & FFT3D_MX_NORMALIZE) ? 1.0 / ((double)n0 * n1 * n2) : 1.0;
     p->tune_us = 0.0;
     if (hipGetDevice(&p->device) != hipSuccess) p->device = 0;
+    if (hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device) !=
+            hipSuccess || p->cus < 1)
+        p->cus = 1;
 
     std::vector<unsigned char> img;
     bank_host_image(precision, img);
