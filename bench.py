@@ -20,6 +20,9 @@ volumes per geometry against
 numpy.fft in float64, and for the matrix-core kernels the executed MFMA TFLOP/s against the
 2.5 PFLOP/s dense peak.
 
+"small_batch_latency": batch-1 complex128 executes of every cube geometry, device time back
+to back and host round trip, launched directly and replayed from a captured graph.
+
 "serving": the request server under Poisson arrivals (20,000 requests, the mixed stream of
 opt-fft_amd/serving.py, 64 GB budget, 16 streams, 30 us batching window for the
 continuously batched server): the highest arrival rate whose p99 latency stays
@@ -503,6 +506,69 @@ def serving(pkg, n=20000, p99_cap_ms=2.0, streams=16, window_us=30.0):
     return out
 
 
+LATENCY_CASES = [((6, 6, 6), -1), ((8, 8, 8), -1), ((12, 12, 12), -1), ((17, 17, 17), -1),
+                 ((24, 24, 24), -1), ((36, 36, 36), -1)]
+
+
+def small_batch_latency(pkg, reps=300):
+    """Batch-1 complex128 executes: the device time per execute back to back (median of
+    3 windows of >= 10 ms) and the host round trip of one execute followed by a stream
+    synchronisation (median of `reps`), launched directly and replayed from a graph captured
+    once per (in, out) pair (the reference replays graph execs for small batches,
+    bench/gpu/impl_4/L17_dmma.cu:1002-1010)."""
+    f, h = pkg.fft3d, pkg.hip
+    out = {}
+    for shape, d in LATENCY_CASES:
+        V = shape[0] * shape[1] * shape[2]
+        x = make_input(shape, 1, seed=5, dist="uniform")
+        bi, bo = h.DeviceBuffer(16 * V), h.DeviceBuffer(16 * V)
+        bi.upload(x)
+        p = f.Plan(shape, 1, d, f.C128)
+        s = h.Stream()
+        g = h.GraphExec(s, lambda st: p.execute(bi.addr, bo.addr, stream=st))
+        res = {"kernel": p.kernel}
+        for mode, launch in (("direct", lambda: p.execute(bi.addr, bo.addr, stream=s)),
+                             ("graph", lambda: g.launch(s))):
+            for _ in range(20):
+                launch()
+            s.synchronize()
+            rt = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                launch()
+                s.synchronize()
+                rt.append(time.perf_counter() - t0)
+            e0, e1 = h.Event(), h.Event()
+            e0.record(s)
+            for _ in range(50):
+                launch()
+            e1.record(s)
+            e1.synchronize()
+            n = max(50, math.ceil(10.0 / (e0.elapsed_ms(e1) / 50)))
+            evs = [h.Event() for _ in range(4)]
+            evs[0].record(s)
+            for w in range(3):
+                for _ in range(n):
+                    launch()
+                evs[w + 1].record(s)
+            evs[-1].synchronize()
+            ws = sorted(evs[w].elapsed_ms(evs[w + 1]) / n * 1e3 for w in range(3))
+            for e in evs + [e0, e1]:
+                e.close()
+            res[mode] = dict(device_us_per_execute=ws[1],
+                             round_trip_us_median=float(np.median(rt)) * 1e6,
+                             round_trip_us_p99=float(np.percentile(rt, 99)) * 1e6)
+        y = bo.download(np.complex128, x.shape)
+        res["rel_l2_vs_numpy"] = _rel_l2(y, _np_dft(x, d))
+        g.close()
+        p.close()
+        s.close()
+        bi.close()
+        bo.close()
+        out["x".join(map(str, shape))] = res
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -515,6 +581,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-latency", action="store_true")
     ap.add_argument("--workload", default=None,
                     help="time only this secondary workload (e.g. L8_c128_fwd_b4096) and print "
                          "its measurement; for profiling runs, no JSON line")
@@ -553,6 +620,9 @@ def main():
     precision = None if a.no_precision else precision_mix(pkg)
     if precision:
         print(f"# precision variants: {precision}", file=sys.stderr, flush=True)
+    latency = None if a.no_latency else small_batch_latency(pkg)
+    if latency:
+        print(f"# batch-1 latency: {latency}", file=sys.stderr, flush=True)
     serve = None if a.no_serving else serving(pkg)
     V = 17 ** 3
     line = {
@@ -590,6 +660,7 @@ def main():
         "cpu_baseline": cpu,
         "secondary": secondary,
         "precision_variants": precision,
+        "small_batch_latency": latency,
         "serving": serve,
     }
     print(json.dumps(line), flush=True)

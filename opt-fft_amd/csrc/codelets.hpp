@@ -11,6 +11,9 @@
 //              the Ruritanian input map n = (N2 n1 + N1 n2) mod N and CRT output map
 //              k = (E1 k1 + E2 k2) mod N are pure register renaming
 //              (index maps as in bench/geom/strategies/L36_pfa.md:10-24)
+// The fourth parameter (V17) selects the line algorithm: for 17 as below; for the composite
+// lengths 1 takes the factors in the other order (8 = 4x2, 16 = 2x8, 6 = 3x2, 12 = 3x4,
+// 24 = 3x8, 36 = 9x4, 48 = 3x16), a radix order the create-time race can choose.
 //   17         V17 = 0: Rader with generator 3 -- a 16-point cyclic convolution done as two
 //              forward DFT16s and a pointwise product with the plan-time spectrum
 //              (construction of bench/geom/strategies/L17_rader.md:21-55);
@@ -273,24 +276,31 @@ FX_HD void dft(cx<T>* x, const Bank<T>* __restrict__ bk)
     } else if constexpr (N == 4) {
         dft4<S>(x);
     } else if constexpr (N == 6) {
-        pfa<2, 3, S>(x, bk);
+        if constexpr (V17 == 0) pfa<2, 3, S>(x, bk);
+        else pfa<3, 2, S>(x, bk);
     } else if constexpr (N == 8) {
-        ct<2, 4, S>(x, bk);
+        if constexpr (V17 == 0) ct<2, 4, S>(x, bk);
+        else ct<4, 2, S>(x, bk);
     } else if constexpr (N == 9) {
         ct<3, 3, S>(x, bk);
     } else if constexpr (N == 12) {
-        pfa<4, 3, S>(x, bk);
+        if constexpr (V17 == 0) pfa<4, 3, S>(x, bk);
+        else pfa<3, 4, S>(x, bk);
     } else if constexpr (N == 16) {
-        ct<4, 4, S>(x, bk);
+        if constexpr (V17 == 0) ct<4, 4, S>(x, bk);
+        else ct<2, 8, S>(x, bk);
     } else if constexpr (N == 17) {
         if constexpr (V17 == 0) rader17<S>(x, bk);
         else fold17<S>(x, bk);
     } else if constexpr (N == 24) {
-        pfa<8, 3, S>(x, bk);
+        if constexpr (V17 == 0) pfa<8, 3, S>(x, bk);
+        else pfa<3, 8, S>(x, bk);
     } else if constexpr (N == 36) {
-        pfa<4, 9, S>(x, bk);
+        if constexpr (V17 == 0) pfa<4, 9, S>(x, bk);
+        else pfa<9, 4, S>(x, bk);
     } else if constexpr (N == 48) {
-        pfa<16, 3, S>(x, bk);
+        if constexpr (V17 == 0) pfa<16, 3, S>(x, bk);
+        else pfa<3, 16, S>(x, bk);
     } else {
         static_assert(N == 1, "unsupported line length");
     }

@@ -22,6 +22,21 @@ namespace fx {
     FX_P(N0, N1, N2, T, +1, VPW, NT, PZ, P,                                                 \
          "pipe_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_v" #VPW "t" #NT "p" #P)
 
+// Autotuning dimensions beyond the tiling: radix order ("_r1": the line factors in the other
+// order, FusedCfg V17 = 1) and LDS padding ("_z0": rows unpadded, PZ = 0).
+#define FX_FV(N0, N1, N2, T, S, VPW, NT, PZ, ALG, NAME) \
+    r.push_back(fused_desc<FusedCfg<N0, N1, N2, T, S, VPW, NT, PZ, 0, ALG>>(NAME))
+#define FX_FV_BOTH(N0, N1, N2, T, TAG, VPW, NT, PZ, ALG, SUF)                                     \
+    FX_FV(N0, N1, N2, T, -1, VPW, NT, PZ, ALG, "fused_" #N0 "x" #N1 "x" #N2 "_" TAG "_fwd_v" #VPW "t" #NT SUF); \
+    FX_FV(N0, N1, N2, T, +1, VPW, NT, PZ, ALG, "fused_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_v" #VPW "t" #NT SUF)
+#define FX_PV(N0, N1, N2, T, S, VPW, NT, PZ, P, ALG, NAME) \
+    r.push_back(fpipe_desc<FusedCfg<N0, N1, N2, T, S, VPW, NT, PZ, 0, ALG>, P>(NAME))
+#define FX_PV_BOTH(N0, N1, N2, T, TAG, VPW, NT, PZ, P, ALG, SUF)                                  \
+    FX_PV(N0, N1, N2, T, -1, VPW, NT, PZ, P, ALG,                                                 \
+          "pipe_" #N0 "x" #N1 "x" #N2 "_" TAG "_fwd_v" #VPW "t" #NT "p" #P SUF);                  \
+    FX_PV(N0, N1, N2, T, +1, VPW, NT, PZ, P, ALG,                                                 \
+          "pipe_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_v" #VPW "t" #NT "p" #P SUF)
+
 #define FX_S(T, S, W, PERM, NAME) r.push_back(shfl8_desc<Shfl8Cfg<T, S, W, PERM>>(NAME))
 #define FX_S_BOTH(T, TAG, W, PERM, ORD)                                            \
     FX_S(T, -1, W, PERM, "shfl_8x8x8_" TAG "_fwd_w" #W ORD);                       \
@@ -35,6 +50,7 @@ void register_fused_small(std::vector<KernelDesc>& r)
     FX_P_BOTH(6, 6, 6, double, "c128", 32, 576, 1, 4);
     FX_F_BOTH(6, 6, 6, double, "c128", 16, 576, 1);
     FX_P_BOTH(6, 6, 6, float, "c64", 32, 576, 1, 4);
+    FX_PV_BOTH(6, 6, 6, double, "c128", 16, 576, 1, 4, 1, "_r1");
     FX_P_BOTH(6, 6, 6, float, "c64", 16, 576, 1, 0);
     FX_F_BOTH(6, 6, 6, float, "c64", 16, 576, 1);
     // L = 8: 64 lines per volume per pass.
@@ -48,6 +64,9 @@ void register_fused_small(std::vector<KernelDesc>& r)
     FX_P_BOTH(8, 8, 8, float, "c64", 4, 256, 1, 8);
     FX_P_BOTH(8, 8, 8, float, "c64", 8, 512, 1, 4);
     FX_F_BOTH(8, 8, 8, float, "c64", 4, 256, 1);
+    FX_FV_BOTH(8, 8, 8, double, "c128", 8, 512, 1, 1, "_r1");
+    FX_FV_BOTH(8, 8, 8, double, "c128", 8, 512, 0, 0, "_z0");
+    FX_FV_BOTH(8, 8, 8, float, "c64", 4, 256, 1, 1, "_r1");
     // Register-resident, shuffles for y and z: W waves (volumes) per workgroup; "n" stores
     // in natural lane order after one more shuffle, "r" from the bit-reversed lanes.
     FX_S_BOTH(double, "c128", 4, 1, "n");
@@ -61,6 +80,9 @@ void register_fused_small(std::vector<KernelDesc>& r)
     FX_F_BOTH(12, 12, 12, double, "c128", 1, 192, 1);
     FX_F_BOTH(12, 12, 12, double, "c128", 4, 576, 1);
     FX_P_BOTH(12, 12, 12, float, "c64", 4, 576, 1, 4);
+    FX_PV_BOTH(12, 12, 12, double, "c128", 4, 576, 1, 2, 1, "_r1");
+    FX_PV_BOTH(12, 12, 12, float, "c64", 4, 576, 1, 4, 1, "_r1");
+    FX_PV_BOTH(12, 12, 12, float, "c64", 4, 576, 0, 4, 0, "_z0");
     FX_P_BOTH(12, 12, 12, float, "c64", 2, 320, 1, 4);
     FX_F_BOTH(12, 12, 12, float, "c64", 4, 576, 1);
     // 8 x 8 x 16: x lines 128 per volume.
@@ -68,6 +90,7 @@ void register_fused_small(std::vector<KernelDesc>& r)
     FX_P_BOTH(8, 8, 16, double, "c128", 4, 512, 1, 2);
     FX_F_BOTH(8, 8, 16, double, "c128", 2, 256, 1);
     FX_P_BOTH(8, 8, 16, float, "c64", 4, 512, 1, 4);
+    FX_PV_BOTH(8, 8, 16, double, "c128", 2, 256, 1, 4, 1, "_r1");
     FX_F_BOTH(8, 8, 16, float, "c64", 4, 512, 1);
 }
 

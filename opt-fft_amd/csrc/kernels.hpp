@@ -412,12 +412,15 @@ __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::W
 // persistent pipelined k_spipe with at most P resident workgroups per CU.
 // ST = 1: k_spipe staggers its HBM traffic as PipeCfg ST does (y-lines of the next group
 // issued in two halves, half of the workgroups starting ~1.6 us late).
+// ALG: line algorithm of the codelets (dft's fourth parameter; 1 = factors in the other order).
+// XH: hints of pass B, bit 0 its loads, bit 1 its stores (nontemporal when set).
 template <int N0_, int N1_, int N2_, typename T_, int S_, int SPW_, int NT_, int PZ_,
-          int NTB_, int CHUNK_, int PIPE_ = 0, int ST_ = 0>
+          int NTB_, int CHUNK_, int PIPE_ = 0, int ST_ = 0, int ALG_ = 0, int XH_ = 3>
 struct PencilCfg {
     using T = T_;
     static constexpr int N0 = N0_, N1 = N1_, N2 = N2_, S = S_, SPW = SPW_, NT = NT_;
     static constexpr int PZ = PZ_, NTB = NTB_, CHUNK = CHUNK_, PIPE = PIPE_, ST = ST_;
+    static constexpr int ALG = ALG_, XH = XH_;
     static constexpr int SPLIT = ST_ ? (N1_ + 1) / 2 : N1_;  // y points in the first half
     static constexpr int V = N0 * N1 * N2;
     static constexpr int PLANE = N1 * N2;
@@ -457,7 +460,7 @@ struct Pencil {
     // Pass A, phase 2: z lines (s, y).
     static FX_HD void zpass(int tid, long bid, long nslab, cx<T>* sm, const Bank<T>* __restrict__ bk)
     {
-        lds_pass<C::N2, 1, C::S, 0, C::NT>(
+        lds_pass<C::N2, 1, C::S, C::ALG, C::NT>(
             tid, sm, nslabs(bid, nslab) * C::N1,
             [](int l) {
                 const int s = l / C::N1;
@@ -479,7 +482,7 @@ struct Pencil {
             cx<T> t[C::N1];
 #pragma unroll
             for (int j = 0; j < C::N1; ++j) t[j] = sm[b + j * C::YS];
-            dft<C::N1, C::S, T>(t, bk);
+            dft<C::N1, C::S, T, C::ALG>(t, bk);
             cx<T>* __restrict__ o = dst + (long)s * C::PLANE + z;
 #pragma unroll
             for (int k = 0; k < C::N1; ++k) o[k * C::N2] = t[k];
@@ -495,14 +498,14 @@ struct Pencil {
         cx<T>* __restrict__ p = buf + v * C::V + q;
         cx<T> t[C::N0];
 #pragma unroll
-        for (int j = 0; j < C::N0; ++j) t[j] = ld_stream(p + j * C::PLANE);
-        dft<C::N0, C::S, T>(t, bk);
+        for (int j = 0; j < C::N0; ++j) t[j] = ld_hint<C::XH & 1>(p + j * C::PLANE);
+        dft<C::N0, C::S, T, C::ALG>(t, bk);
         if (scl != T(1)) {
 #pragma unroll
-            for (int k = 0; k < C::N0; ++k) st_stream(p + k * C::PLANE, scale(t[k], scl));
+            for (int k = 0; k < C::N0; ++k) st_hint<C::XH & 2>(p + k * C::PLANE, scale(t[k], scl));
         } else {
 #pragma unroll
-            for (int k = 0; k < C::N0; ++k) st_stream(p + k * C::PLANE, t[k]);
+            for (int k = 0; k < C::N0; ++k) st_hint<C::XH & 2>(p + k * C::PLANE, t[k]);
         }
     }
 };
@@ -561,7 +564,7 @@ struct SlabPipe {
         for (int i = 0; i < LPT; ++i) {
             const int l = tid + i * C::NT;
             if (l < nl) {
-                dft<C::N1, C::S, T>(R[i], bk);
+                dft<C::N1, C::S, T, C::ALG>(R[i], bk);
                 const int s = l / C::N2;
                 const int z = l - s * C::N2;
 #pragma unroll

@@ -35,6 +35,12 @@ def lib():
         h.hipGetErrorString.argtypes = [ctypes.c_int]
         h.hipGetErrorString.restype = ctypes.c_char_p
         h.hipMemGetInfo.argtypes = [ctypes.POINTER(sz), ctypes.POINTER(sz)]
+        h.hipStreamBeginCapture.argtypes = [vp, ctypes.c_int]
+        h.hipStreamEndCapture.argtypes = [vp, ctypes.POINTER(vp)]
+        h.hipGraphInstantiate.argtypes = [ctypes.POINTER(vp), vp, vp, vp, sz]
+        h.hipGraphLaunch.argtypes = [vp, vp]
+        h.hipGraphExecDestroy.argtypes = [vp]
+        h.hipGraphDestroy.argtypes = [vp]
         _hip = h
     return _hip
 
@@ -149,3 +155,30 @@ class Event:
         if self.handle is not None and self.handle.value:
             lib().hipEventDestroy(self.handle)
         self.handle = None
+
+
+class GraphExec:
+    """The work that `enqueue(stream)` puts on `stream`, captured once into a graph and
+    instantiated; launch() replays it."""
+
+    def __init__(self, stream, enqueue):
+        L = lib()
+        self.graph, self.exec = ctypes.c_void_p(), ctypes.c_void_p()
+        check(L.hipStreamBeginCapture(stream.handle, 1), "hipStreamBeginCapture")  # thread-local
+        try:
+            enqueue(stream)
+        finally:
+            check(L.hipStreamEndCapture(stream.handle, ctypes.byref(self.graph)),
+                  "hipStreamEndCapture")
+        check(L.hipGraphInstantiate(ctypes.byref(self.exec), self.graph, None, None, 0),
+              "hipGraphInstantiate")
+
+    def launch(self, stream):
+        check(lib().hipGraphLaunch(self.exec, stream.handle), "hipGraphLaunch")
+
+    def close(self):
+        if self.exec is not None and self.exec.value:
+            lib().hipGraphExecDestroy(self.exec)
+        if self.graph is not None and self.graph.value:
+            lib().hipGraphDestroy(self.graph)
+        self.exec = self.graph = None
