@@ -278,8 +278,8 @@ def cpu_baseline():
 
 # Precision variants on uneven batches (complex64): one persistent mixed launch, the
 # per-geometry complex64 FFT plans, and the DFT-as-matrix variant on the matrix cores.
-C4_ITEMS = [((6, 6, 6), 65536), ((8, 8, 8), 32768), ((12, 12, 12), 8192), ((17, 17, 17), 2048)]
-C4_CHECK = 256          # volumes per item compared with numpy.fft in float64
+UNEVEN_MIX = [((6, 6, 6), 65536), ((8, 8, 8), 32768), ((12, 12, 12), 8192), ((17, 17, 17), 2048)]
+MIX_CHECK = 256          # volumes per item compared with numpy.fft in float64
 PEAK_MFMA_TFLOPS = 2500.0   # dense bf16/fp16 matrix peak of one MI355X
 
 
@@ -315,7 +315,7 @@ def mfma_flops(shape, nvol, kernel=""):
 def precision_mix(pkg, steps=20):
     f, h = pkg.fft3d, pkg.hip
     rng = np.random.default_rng(44)
-    xs = [_uniform_c64(rng, sh, b) for sh, b in C4_ITEMS]
+    xs = [_uniform_c64(rng, sh, b) for sh, b in UNEVEN_MIX]
     nbytes = sum(x.nbytes for x in xs)
     # Input/output sets rotated so that >= 2 GiB pass between two uses of one set: the
     # mix (882 MB in + out) and each item alone would otherwise stay in the 256 MB
@@ -359,15 +359,15 @@ def precision_mix(pkg, steps=20):
 
     def errors(bufs):
         out = {}
-        for (sh, b), x, ob in zip(C4_ITEMS, xs, bufs):
-            y = ob.download(np.complex64, x.shape)[:C4_CHECK]
-            ref = _np_dft(x[:C4_CHECK])
+        for (sh, b), x, ob in zip(UNEVEN_MIX, xs, bufs):
+            y = ob.download(np.complex64, x.shape)[:MIX_CHECK]
+            ref = _np_dft(x[:MIX_CHECK])
             out[f"L{sh[0]}"] = dict(rel_l2=_rel_l2(y, ref), rel_max=_rel_max(y, ref))
         return out
 
-    res = {"items": [dict(shape=list(sh), batch=b) for sh, b in C4_ITEMS],
-           "bytes_per_launch": 2 * nbytes, "error_sample_volumes": C4_CHECK}
-    mp = f.MultiPlan([(sh, b, -1) for sh, b in C4_ITEMS], f.C64)
+    res = {"items": [dict(shape=list(sh), batch=b) for sh, b in UNEVEN_MIX],
+           "bytes_per_launch": 2 * nbytes, "error_sample_volumes": MIX_CHECK}
+    mp = f.MultiPlan([(sh, b, -1) for sh, b in UNEVEN_MIX], f.C64)
     t = timed(lambda s_: mp.execute([b.addr for b in ins[s_]], [b.addr for b in outs[s_]],
                                     stream=stream), sets)
     h.synchronize()
@@ -375,11 +375,11 @@ def precision_mix(pkg, steps=20):
                                      errors=errors(outs[0]))
     mp.close()
     for tag, flag in (("one_launch_mfma_bf16x3", f.MFMA_BF16), ("one_launch_mfma_fp16x3", f.MFMA_FP16)):
-        mp = f.MultiPlan([(sh, b, -1) for sh, b in C4_ITEMS], f.C64, flags=flag)
+        mp = f.MultiPlan([(sh, b, -1) for sh, b in UNEVEN_MIX], f.C64, flags=flag)
         t = timed(lambda s_: mp.execute([b.addr for b in ins[s_]], [b.addr for b in outs[s_]],
                                         stream=stream), sets)
         h.synchronize()
-        fl = sum(mfma_flops(sh, b, mp.info()[0]) for sh, b in C4_ITEMS)
+        fl = sum(mfma_flops(sh, b, mp.info()[0]) for sh, b in UNEVEN_MIX)
         res[tag] = dict(kernel=mp.info()[0], us=t * 1e6, GBs=2 * nbytes / t / 1e9,
                         mfma_tflops=fl / t / 1e12, mfma_util=fl / t / 1e12 / PEAK_MFMA_TFLOPS,
                         errors=errors(outs[0]))
@@ -390,21 +390,21 @@ def precision_mix(pkg, steps=20):
     # Per-geometry plans: each item with its own rotation of >= 2 GiB.
     for tag in ("fft_c64", "mfma_bf16x3", "mfma_fp16x3"):
         res[tag] = {}
-    for (sh, b), x in zip(C4_ITEMS, xs):
+    for (sh, b), x in zip(UNEVEN_MIX, xs):
         nrot = math.ceil(ROTATE_BYTES / (2 * x.nbytes)) + 1
         bi = [h.DeviceBuffer(x.nbytes) for _ in range(nrot)]
         bo = [h.DeviceBuffer(x.nbytes) for _ in range(nrot)]
         bi[0].upload(x)
         for q in bi[1:]:
             h.check(h.lib().hipMemcpy(q.ptr, bi[0].ptr, x.nbytes, h.D2D), "hipMemcpy D2D")
-        ref = _np_dft(x[:C4_CHECK])
+        ref = _np_dft(x[:MIX_CHECK])
         for tag, flag in (("fft_c64", 0), ("mfma_bf16x3", f.MFMA_BF16),
                           ("mfma_fp16x3", f.MFMA_FP16)):
             p = f.Plan(sh, b, f.FORWARD, f.C64, flags=flag)
             tk = timed(lambda s_: p.execute(bi[s_].addr, bo[s_].addr, stream=stream), nrot)
             h.synchronize()
             d = dict(kernel=p.kernel, us=tk * 1e6, GBs=2 * x.nbytes / tk / 1e9, rotate_sets=nrot)
-            y = bo[0].download(np.complex64, x.shape)[:C4_CHECK]
+            y = bo[0].download(np.complex64, x.shape)[:MIX_CHECK]
             d["rel_l2"] = _rel_l2(y, ref)
             d["rel_max"] = _rel_max(y, ref)
             if flag:

@@ -155,11 +155,11 @@ void fft3d_mx_multi_destroy(fft3d_mx_multi *m);
  * forbid allocation).  Same transform as a plan of (n0, n1, n2, batch, direction,
  * precision) created with FFT3D_MX_NO_TUNE, enqueued on `stream`, with no plan: the
  * constant tables and the claim counters the kernels use are static device arrays of the
- * library (about 10 KiB per device, held until the process ends).  flags: 0 or
+ * library (about 134 KiB per device, held until the process ends).  flags: 0 or
  * FFT3D_MX_NORMALIZE.  0 <= batch <= 2^31 - 1 (claim tickets are 32-bit); other sizes are
  * refused with FFT3D_MX_EINVAL.  The first call with a device current sets that device up
  * synchronously and must not be captured; a call with batch 0 does only that.  All calls
- * on one device share its 64 claim counters: up to 64 calls may be in flight on a device
+ * on one device share its 64 sets of claim counters: up to 64 calls may be in flight on a device
  * at once, and calls beyond that overlap on a counter and race.  0 or an error code. */
 int fft3d_mx_execute_noalloc(int n0, int n1, int n2, long batch, int direction, int precision,
                              unsigned flags, const void *in, void *out, void *stream);

@@ -1,10 +1,11 @@
 // Claim counters of a plan, handed to launches so that no two launches that may run at
-// the same time draw blocks from one counter.
+// the same time draw blocks from the same counters.
 //
 // A persistent kernel draws its blocks from a counter in device memory (claim_block in
 // kernels.hpp) and hands it back at zero when the last ticket is drawn.  Two launches
 // drawing from one counter at once would skip or repeat blocks, so a plan owns three
-// regions of kSlots counters (one 64 B line each) and assigns them as follows:
+// regions of kSlots counter sets (kClaimWords unsigned each, one set per launch) and
+// assigns them as follows:
 //
 //   region 0  while every launch of the plan so far went to one stream.  A stream runs its
 //             launches one after another, so when launch n starts, launch n - kSlots (the
@@ -27,13 +28,15 @@
 
 #include <mutex>
 
+#include "claim_layout.hpp"
+
 namespace fx {
 
 template <class RT>
 class ClaimSlotsT {
 public:
     static constexpr int kSlots = 64;
-    static constexpr int kLine = 16;  // unsigned per 64 B line
+    static constexpr int kLine = kClaimWords;  // unsigned per launch (claim_layout.hpp)
     static constexpr size_t bytes() { return sizeof(unsigned) * 3 * kSlots * kLine; }
 
     ClaimSlotsT() = default;

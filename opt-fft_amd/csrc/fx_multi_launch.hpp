@@ -23,11 +23,12 @@ hipError_t launch_multi(const void* args, const int2* tasks, int ntasks, const v
 }
 
 template <typename T, int PRESET>
-hipError_t launch_multi_dev(const void* d, const void* bank, int grid, hipStream_t stream)
+hipError_t launch_multi_dev(const void* d, const void* bank, unsigned* work, int grid,
+                            hipStream_t stream)
 {
     using M = MultiCfg<T, PRESET>;
     hipLaunchKernelGGL((k_multi_dev<T, PRESET>), dim3((unsigned)grid), dim3(M::NT), 0, stream,
-                       (MultiDescriptor<T>*)d, (const Bank<T>*)bank);
+                       (MultiDescriptor<T>*)d, (const Bank<T>*)bank, work);
     return hipGetLastError();
 }
 
@@ -39,14 +40,14 @@ int multi_dev_grid()
 }
 
 template <typename T, int PRESET>
-MultiDesc multi_desc(const char* name)
+MultiDesc multi_desc(const char* name, int order = 0)
 {
     using M = MultiCfg<T, PRESET>;
     return MultiDesc{prec_of<T>(), PRESET, M::NT,
                      {M::vpw(0), M::vpw(1), M::vpw(2), M::vpw(3), M::vpw(4)},
                      sizeof(cx<T>) * (size_t)M::LDS, name, &launch_multi<T, PRESET>,
                      &launch_multi_dev<T, PRESET>, &multi_dev_grid<T, PRESET>, 0,
-                     &multi_launch_grid<T, PRESET>};
+                     &multi_launch_grid<T, PRESET>, order};
 }
 
 }  // namespace fx

@@ -32,6 +32,7 @@
 // phases are plain __host__ __device__ functions of (thread, block).  Launchers come from
 // <fx_launch.hpp>, found on the include path.
 #pragma once
+#include "claim_layout.hpp"
 #include "codelets.hpp"
 #include "registry.hpp"
 
@@ -359,17 +360,38 @@ struct HintCfg : C {
 template <class C>
 using PlainCfg = HintCfg<C, 0>;
 
-// Next block of a persistent workgroup.  Each workgroup starts at block blockIdx.x and then
-// claims one ticket per iteration until it draws a block >= nblocks, so a launch with
-// grid <= nblocks draws exactly nblocks tickets, 0 .. nblocks-1: the holder of the last one
-// puts the counter back to zero for the next launch.  Claiming hands blocks to whichever
-// workgroup is free first; a fixed stride (work == NULL) lets the slowest CU set the time.
+// Claim queues of a launch of `grid` workgroups over nblocks >= grid blocks: Q =
+// min(kClaimQueues, grid) counters (claim_layout.hpp), block b in queue b % Q, workgroup w
+// drawing from queue w % Q after starting with block w.  Queue q holds nq blocks and gq
+// workgroups; each of them draws tickets until one is past the queue's end, so the queue
+// hands out exactly nq tickets, 0 .. nq-1 (nq - gq blocks and one refusal per workgroup),
+// and the holder of the last one puts its counter back to zero for the next launch.
+struct ClaimQueue {
+    long q, nqueues, gq, nq;
+};
+FX_HD inline ClaimQueue claim_queue(long w, long grid, long nblocks)
+{
+    const long Q = grid < kClaimQueues ? grid : kClaimQueues;
+    const long q = w % Q;
+    return ClaimQueue{q, Q, (grid - q + Q - 1) / Q, (nblocks - q + Q - 1) / Q};
+}
+// Block of ticket t of queue c; >= nblocks once the queue is exhausted.
+FX_HD inline long claim_queue_block(const ClaimQueue& c, long t)
+{
+    return c.q + c.nqueues * (c.gq + t);
+}
+
+// Next block of a persistent workgroup (first block: blockIdx.x).  Claiming hands blocks
+// to whichever workgroup of the queue is free first; a fixed stride (work == NULL) lets the
+// slowest CU set the time.  `work`: kClaimWords zeroed counters, left at zero.
 __device__ __forceinline__ long claim_block(unsigned* work, long nblocks, long b)
 {
     if (!work) return b + gridDim.x;
-    const unsigned t = atomicAdd(work, 1u);
-    if ((long)t == nblocks - 1) (void)atomicExch(work, 0u);
-    return (long)gridDim.x + (long)t;
+    const ClaimQueue c = claim_queue(blockIdx.x, gridDim.x, nblocks);
+    unsigned* ctr = work + c.q * kClaimLine;
+    const unsigned t = atomicAdd(ctr, 1u);
+    if ((long)t == c.nq - 1) (void)atomicExch(ctr, 0u);
+    return claim_queue_block(c, (long)t);
 }
 
 template <class C>

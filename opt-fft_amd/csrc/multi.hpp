@@ -72,8 +72,23 @@ struct MultiPreset<T, 2> {
     using G3 = FusedCfg<17, 17, 17, T, -1, 2, NT, 0, 0, 1>;
     using G4 = FusedCfg<8, 8, 16, T, -1, 9, NT, 1, 0, 1>;
 };
-// (Preset 2 with nontemporal loads, or loads and stores, took 207.8 us on the complex64
-// C4 mix against 186.8 us plain.)
+// 320 threads, every geometry's block sized to the LDS of one 17^3 volume (39.3 KB at
+// complex64, 78.6 KB at complex128): four (complex64) or two (complex128) resident
+// workgroups per CU, each with a whole block of its own.
+template <typename T>
+struct MultiPreset<T, 3> {
+    static constexpr int NT = 320;
+    static constexpr int HN = 0;
+    using G0 = FusedCfg<6, 6, 6, T, -1, 19, NT, 1, 0, 1>;
+    using G1 = FusedCfg<8, 8, 8, T, -1, 8, NT, 1, 0, 1>;
+    using G2 = FusedCfg<12, 12, 12, T, -1, 2, NT, 1, 0, 1>;
+    using G3 = FusedCfg<17, 17, 17, T, -1, 1, NT, 0, 0, 1>;
+    using G4 = FusedCfg<8, 8, 16, T, -1, 4, NT, 1, 0, 1>;
+};
+// Measured on the complex64 mix 6^3 x 65536, 8^3 x 32768, 12^3 x 8192, 17^3 x 2048 (not
+// built): preset 3 with the items interleaved and nontemporal stores 168.6 us, with
+// nontemporal loads, or loads and stores, 174.0 us, against 168.9 us plain on one device;
+// preset 2 with nontemporal loads, or loads and stores, 207.8 us against 186.8 us plain.
 // (The same blocks on 1024 threads, 32 waves per CU, spill at the 64-VGPR cap: not built.)
 
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
@@ -171,36 +186,36 @@ __global__ void __launch_bounds__((MultiCfg<T, PRESET>::NT))
 
 // Same kernel with the items, the task count and the task list read from device memory, so a
 // launch can be captured once in a hipGraph and replayed with new descriptors (serving).
-// `work` is the launch's claim counter: zero in every descriptor the host writes, and left
-// at zero by the launch (claim_block).
+// `work`: the launch's claim counters (kClaimWords, zero before the first launch and left
+// at zero by every launch, claim_block), owned by whoever owns the descriptor.
 template <typename T>
 struct MultiDescriptor {
     MultiArgs<T> args;
     int ntasks;
-    unsigned work;
     int2 tasks[1];  // ntasks entries follow
 };
 
 template <typename T, int PRESET>
 __global__ void __launch_bounds__((MultiCfg<T, PRESET>::NT))
     __attribute__((amdgpu_waves_per_eu(MultiCfg<T, PRESET>::WPE, 8)))
-    k_multi_dev(MultiDescriptor<T>* __restrict__ d, const Bank<T>* __restrict__ bk)
+    k_multi_dev(MultiDescriptor<T>* __restrict__ d, const Bank<T>* __restrict__ bk,
+                unsigned* __restrict__ work)
 {
     using M = MultiCfg<T, PRESET>;
     __shared__ cx<T> sm[M::LDS];
     __shared__ long s_next;
     const int ntasks = d->ntasks;
-    unsigned* work = ntasks >= 2L * gridDim.x ? &d->work : nullptr;
-    multi_walk<T, PRESET>(d->args, d->tasks, ntasks, bk, work, sm, &s_next);
+    multi_walk<T, PRESET>(d->args, d->tasks, ntasks, bk, ntasks >= 2L * gridDim.x ? work : nullptr,
+                          sm, &s_next);
 }
 
 // Registry entry of a multi kernel.
 // `work`: claim counter as for LaunchFn (registry.hpp).
 typedef hipError_t (*MultiLaunchFn)(const void* args, const int2* tasks, int ntasks,
                                     const void* bank, unsigned* work, hipStream_t stream);
-// Launch of the descriptor-reading form on `grid` workgroups.
-typedef hipError_t (*MultiDevLaunchFn)(const void* descriptor, const void* bank, int grid,
-                                       hipStream_t stream);
+// Launch of the descriptor-reading form on `grid` workgroups with claim counters `work`.
+typedef hipError_t (*MultiDevLaunchFn)(const void* descriptor, const void* bank, unsigned* work,
+                                       int grid, hipStream_t stream);
 struct MultiDesc {
     int prec;
     int preset;
@@ -213,6 +228,7 @@ struct MultiDesc {
     int (*resident_grid)();  // persistent grid of the descriptor form on the current device
     int algo = 0;            // as KernelDesc::algo: 0 FFT, 1 / 2 matrix-core bf16 / fp16
     long (*launch_grid)(long ntasks) = nullptr;  // workgroups `launch` starts for ntasks
+    int order = 0;  // task order (multi_tasks): 0 heaviest first, 1 items interleaved
 };
 
 void register_multi(std::vector<MultiDesc>& r);

@@ -31,10 +31,10 @@
 using namespace fx;
 
 constexpr int kNoallocSlots = 64;
-constexpr int kLineU = 16;  // unsigned per 64 B counter line
+constexpr int kLineU = kClaimWords;  // claim counters of one call (claim_layout.hpp)
 constexpr int kMaxDevices = 64;
 
-// Claim counters of every call on this device (kNoallocSlots lines of 64 B, 4 KiB).
+// Claim counters of every call on this device (kNoallocSlots sets of 2 KiB, 128 KiB).
 static __device__ __align__(256) unsigned scratch[kNoallocSlots * kLineU];
 // Constant banks, one per precision, uploaded at set-up.
 static __device__ __align__(256) Bank<double> bank_c128;

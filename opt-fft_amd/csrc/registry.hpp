@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <vector>
 
+#include "claim_layout.hpp"
+
 namespace fx {
 
 enum Prec { kC128 = 0, kC64 = 1 };
@@ -13,10 +15,11 @@ enum Kind { kFused = 0, kPencil = 1 };
 
 // Launch on `stream`: transform nvol contiguous volumes from `in` to `out` (device
 // pointers, out-of-place), multiplying every output by `scale`.  `bank` is the device
-// Bank<T> of matching precision.  `work` is a device claim counter that is zero when the
-// launch starts and that the launch leaves at zero (persistent kernels hand out blocks
-// from it); launches that share one counter must be ordered on one stream.  NULL makes
-// persistent kernels walk their blocks in a fixed order instead.  Returns the launch status.
+// Bank<T> of matching precision.  `work` points to kClaimWords device claim counters
+// (claim_layout.hpp) that are zero when the launch starts and that the launch leaves at
+// zero (persistent kernels hand out blocks from them); launches that share counters must be
+// ordered on one stream.  NULL makes persistent kernels walk their blocks in a fixed order
+// instead.  Returns the launch status.
 typedef hipError_t (*LaunchFn)(const void* in, void* out, long nvol, const void* bank,
                                double scale, unsigned* work, hipStream_t stream);
 

@@ -95,11 +95,13 @@ struct Req {
     std::atomic<int> done{0};
 };
 
-// A reusable mixed launch: device descriptor, pinned staging, captured graph.
+// A reusable mixed launch: device descriptor, pinned staging, claim counters (zeroed once,
+// left at zero by every launch), captured graph.
 struct Slot {
     int prec;
     void* dev = nullptr;
     void* host = nullptr;
+    unsigned* work = nullptr;
     size_t bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -190,7 +192,6 @@ static int launch_mixed(fft3d_srv* s, std::vector<Req*>& reqs, int stream)
             it.scale = (T)1;
         }
         desc->ntasks = (int)tasks.size();
-        desc->work = 0u;
         std::memcpy(desc->tasks, tasks.data(), tasks.size() * sizeof(int2));
         return offsetof(std::remove_reference_t<decltype(*desc)>, tasks) + tasks.size() * sizeof(int2);
     };
@@ -423,7 +424,9 @@ static bool make_slots(fft3d_srv* s, int prec, int count)
         sl.prec = prec;
         sl.bytes = bytes;
         ok = hipMalloc(&sl.dev, bytes) == hipSuccess &&
-             hipHostMalloc(&sl.host, bytes, hipHostMallocDefault) == hipSuccess;
+             hipHostMalloc(&sl.host, bytes, hipHostMallocDefault) == hipSuccess &&
+             hipMalloc((void**)&sl.work, kClaimWords * sizeof(unsigned)) == hipSuccess &&
+             hipMemset(sl.work, 0, kClaimWords * sizeof(unsigned)) == hipSuccess;
         if (ok) {
             // An empty descriptor until the first refresh: every workgroup exits at once.
             std::memset(sl.host, 0, bytes);
@@ -431,7 +434,7 @@ static bool make_slots(fft3d_srv* s, int prec, int count)
         }
         ok = ok && hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (ok) {
-            const hipError_t le = d->launch_dev(sl.dev, s->bank[prec], grid, cap);
+            const hipError_t le = d->launch_dev(sl.dev, s->bank[prec], sl.work, grid, cap);
             const hipError_t ce = hipStreamEndCapture(cap, &sl.graph);
             ok = le == hipSuccess && ce == hipSuccess &&
                  hipGraphInstantiate(&sl.exec, sl.graph, nullptr, nullptr, 0) == hipSuccess;
@@ -448,6 +451,7 @@ static void destroy_all(fft3d_srv* s)
         if (sl.exec) (void)hipGraphExecDestroy(sl.exec);
         if (sl.graph) (void)hipGraphDestroy(sl.graph);
         (void)hipFree(sl.dev);
+        (void)hipFree(sl.work);
         if (sl.host) (void)hipHostFree(sl.host);
     }
     for (auto& kv : s->plans) fft3d_mx_destroy(kv.second);

@@ -37,14 +37,14 @@ struct RT {
     static void event_destroy(int) {}
 };
 
-static unsigned g_dev[3 * 64 * 16];
+static unsigned g_dev[fx::ClaimSlotsT<RT>::bytes() / sizeof(unsigned)];
 static int g_fail = 0;
 #define CHECK(c, ...) do { if (!(c)) { printf("FAIL line %d: ", __LINE__); printf(__VA_ARGS__); printf("\n"); g_fail = 1; } } while (0)
 
 int main()
 {
     using CS = fx::ClaimSlotsT<RT>;
-    auto idx = [](unsigned* w) { return (int)((w - g_dev) / 16); };
+    auto idx = [](unsigned* w) { return (int)((w - g_dev) / CS::kLine); };
     {   // one stream: region 0 round-robin, no events, no waits
         CS c;
         c.attach(g_dev);

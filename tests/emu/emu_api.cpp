@@ -33,13 +33,56 @@ extern "C" int emu_transform(const char* name, const void* in, void* out, long n
         bank_host_image(k.prec, img);
         g_emu_reverse = reverse;
         g_emu_bad_block = -1;
-        unsigned work = 0;  // the launch must hand it back at zero
-        if (k.launch(in, out, nvol, img.data(), scale, &work, nullptr) != hipSuccess) return -2;
+        std::vector<unsigned> work(kClaimWords, 0u);  // the launch must hand them back at zero
+        if (k.launch(in, out, nvol, img.data(), scale, work.data(), nullptr) != hipSuccess)
+            return -2;
         if (g_emu_bad_block >= 0) return (int)(-1000 - g_emu_bad_block);
-        return work == 0 ? 0 : -3;
+        for (unsigned c : work)
+            if (c != 0) return -3;
+        return 0;
     }
     return -1;
 }
+
+// Claim queues at a real grid: `grid` workgroups, each starting at its own block and then
+// claiming (emu_claim, the device arithmetic) until refused, interleaved in a random order.
+// Returns 0 when every block ran exactly once and every counter is back at zero, -1 for a
+// block run twice or out of range, -2 for a block never run, -3 for a counter left non-zero.
+extern "C" int emu_claim_sim(long grid, long nblocks, unsigned seed)
+{
+    if (grid < 1 || nblocks < grid) return -1;
+    std::vector<unsigned> work(kClaimWords, 0u);
+    std::vector<int> runs(nblocks, 0);
+    std::vector<long> cur(grid);
+    std::vector<long> live;
+    for (long w = 0; w < grid; ++w) {
+        cur[w] = w;
+        live.push_back(w);
+    }
+    unsigned r = seed * 2654435761u + 1u;
+    while (!live.empty()) {
+        r = r * 1664525u + 1013904223u;
+        const size_t i = (r >> 8) % live.size();
+        const long w = live[i];
+        if (cur[w] < 0 || cur[w] >= nblocks || runs[cur[w]]++) return -1;
+        const long nb = emu_claim(work.data(), nblocks, grid, w, cur[w]);
+        if (nb >= nblocks) {
+            live[i] = live.back();
+            live.pop_back();
+        } else {
+            cur[w] = nb;
+        }
+    }
+    for (long b = 0; b < nblocks; ++b)
+        if (runs[b] != 1) return -2;
+    for (unsigned c : work)
+        if (c != 0) return -3;
+    return 0;
+}
+
+// Thread and workgroup order of the next emu_multi calls (0 ascending, 1 descending).
+static int g_multi_reverse = 0;
+extern "C" void emu_set_multi_reverse(int r) { g_multi_reverse = r; }
 
 extern "C" int emu_multi_count(void) { return (int)multi_registry().size(); }
 extern "C" const char* emu_multi_name(int i)
@@ -67,12 +110,14 @@ static int emu_multi_t(const MultiDesc& d, int n, const int* shape, const int* b
     multi_tasks(d, code, batch, n, sizeof(cx<T>), tasks);
     std::vector<unsigned char> img;
     bank_host_image(d.prec, img);
-    g_emu_reverse = 0;
+    g_emu_reverse = g_multi_reverse;
     g_emu_bad_block = -1;
-    unsigned work = 0;  // claim counter; the launch must hand it back at zero
-    (void)d.launch(&a, tasks.data(), (int)tasks.size(), img.data(), &work, nullptr);
+    std::vector<unsigned> work(kClaimWords, 0u);  // must all be back at zero afterwards
+    (void)d.launch(&a, tasks.data(), (int)tasks.size(), img.data(), work.data(), nullptr);
     if (g_emu_bad_block >= 0) return (int)(-1000 - g_emu_bad_block);
-    return work == 0 ? 0 : -3;
+    for (unsigned c : work)
+        if (c != 0) return -3;
+    return 0;
 }
 
 extern "C" int emu_multi(const char* name, int n, const int* shape, const int* batch,

@@ -52,14 +52,17 @@ inline void emu_threads(int nt, F f)
         for (int t = 0; t < nt; ++t) f(t);
 }
 
-// Host mirror of the device claim_block (kernels.hpp): the persistent workgroup at block b
-// draws its next block from the launch's counter; the holder of ticket nblocks-1 resets it.
-inline long emu_claim(unsigned* work, long nblocks, long grid, long b)
+// Host mirror of the device claim_block (kernels.hpp, same queue arithmetic): persistent
+// workgroup w at block b draws its next block from its queue's counter; the holder of the
+// queue's last ticket resets it.
+inline long emu_claim(unsigned* work, long nblocks, long grid, long w, long b)
 {
     if (!work) return b + grid;
-    const unsigned t = (*work)++;
-    if ((long)t == nblocks - 1) *work = 0u;
-    return grid + (long)t;
+    const ClaimQueue c = claim_queue(w, grid, nblocks);
+    unsigned* ctr = work + c.q * kClaimLine;
+    const unsigned t = (*ctr)++;
+    if ((long)t == c.nq - 1) *ctr = 0u;
+    return claim_queue_block(c, (long)t);
 }
 
 // Emulated persistent launch: `grid` workgroups, each starting at its own index, advanced
@@ -80,7 +83,7 @@ inline void emu_persistent(long nblocks, long grid, unsigned* work, Start start,
         for (long i = 0; i < grid; ++i) {
             const long w = g_emu_reverse ? grid - 1 - i : i;
             if (!live[w]) continue;
-            const long nb = emu_claim(work, nblocks, grid, cur[w]);
+            const long nb = emu_claim(work, nblocks, grid, w, cur[w]);
             iter(w, cur[w], nb);
             if (nb >= nblocks) {
                 live[w] = 0;

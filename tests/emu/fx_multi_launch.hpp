@@ -28,23 +28,23 @@ hipError_t launch_multi(const void* args, const int2* tasks, int ntasks, const v
 }
 
 template <typename T, int PRESET>
-hipError_t launch_multi_dev(const void* d, const void* bank, int, hipStream_t)
+hipError_t launch_multi_dev(const void* d, const void* bank, unsigned* work, int, hipStream_t)
 {
     MultiDescriptor<T>* md = (MultiDescriptor<T>*)d;
-    return launch_multi<T, PRESET>(&md->args, md->tasks, md->ntasks, bank, &md->work, nullptr);
+    return launch_multi<T, PRESET>(&md->args, md->tasks, md->ntasks, bank, work, nullptr);
 }
 
 template <typename T, int PRESET>
 int multi_dev_grid() { return 3; }
 
 template <typename T, int PRESET>
-MultiDesc multi_desc(const char* name)
+MultiDesc multi_desc(const char* name, int order = 0)
 {
     using M = MultiCfg<T, PRESET>;
     return MultiDesc{prec_of<T>(), PRESET, M::NT,
                      {M::vpw(0), M::vpw(1), M::vpw(2), M::vpw(3), M::vpw(4)},
                      sizeof(cx<T>) * (size_t)M::LDS, name, &launch_multi<T, PRESET>,
-                     &launch_multi_dev<T, PRESET>, &multi_dev_grid<T, PRESET>};
+                     &launch_multi_dev<T, PRESET>, &multi_dev_grid<T, PRESET>, 0, nullptr, order};
 }
 
 }  // namespace fx

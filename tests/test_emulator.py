@@ -134,3 +134,13 @@ def test_mixed_batch_kernels_on_the_emulator(emu):
                 continue
             err = oracle.rel_l2(y, sc * oracle.fftn_numpy(x, sign=d))
             assert err <= (1e-13 if prec == 0 else 2e-6), (name, s, d, err)
+
+
+@pytest.mark.parametrize("grid", [1, 3, 31, 32, 33, 256, 2048])
+def test_claim_queues_hand_out_every_block_once(emu, grid):
+    """The claim queues (device arithmetic, host counters) at real grid sizes: every block
+    runs exactly once and every counter returns to zero, whatever the interleaving."""
+    emu.emu_claim_sim.argtypes = [ctypes.c_long, ctypes.c_long, ctypes.c_uint]
+    for nblocks in sorted({grid, grid + 1, 2 * grid, 2 * grid + 31, 10 * grid + 7}):
+        for seed in range(3):
+            assert emu.emu_claim_sim(grid, nblocks, seed) == 0, (grid, nblocks, seed)

@@ -71,8 +71,8 @@ def test_persistent_variants_walk_many_blocks(gpu, key):
 
 
 # --------------------------------------------------------------------------- uneven mix
-C4 = [((6, 6, 6), 65536), ((8, 8, 8), 32768), ((12, 12, 12), 8192), ((17, 17, 17), 2048)]
-C4_TOL = {0: 2e-6, 4: 2e-5, 8: 6e-7}   # flags -> rel L2 bar
+UNEVEN_MIX = [((6, 6, 6), 65536), ((8, 8, 8), 32768), ((12, 12, 12), 8192), ((17, 17, 17), 2048)]
+MIX_TOL = {0: 2e-6, 4: 2e-5, 8: 6e-7}   # flags -> rel L2 bar
 
 
 def _c64(seed, shape, batch):
@@ -87,18 +87,18 @@ def _energy(a):
 @pytest.mark.parametrize("flags", [0, 4, 8], ids=["fft", "mfma_bf16x3", "mfma_fp16x3"])
 def test_uneven_mix_one_launch_full_size(gpu, flags):
     f = gpu.fft3d
-    tol = C4_TOL[flags]
-    x1 = [_c64(100 + k, s, b) for k, (s, b) in enumerate(C4)]
-    x2 = [_c64(200 + k, s, b) for k, (s, b) in enumerate(C4)]
+    tol = MIX_TOL[flags]
+    x1 = [_c64(100 + k, s, b) for k, (s, b) in enumerate(UNEVEN_MIX)]
+    x2 = [_c64(200 + k, s, b) for k, (s, b) in enumerate(UNEVEN_MIX)]
     a, c = 0.75 - 0.5j, -1.25 + 0.25j
     x3 = [(a * p + c * q).astype(np.complex64) for p, q in zip(x1, x2)]
-    m = f.MultiPlan([(s, b, -1) for s, b in C4], precision=f.C64, flags=flags)
+    m = f.MultiPlan([(s, b, -1) for s, b in UNEVEN_MIX], precision=f.C64, flags=flags)
     name, ntasks, grid = m.info()
     print(f"{name}: {ntasks} tasks on {grid} workgroups", flush=True)
     y1, y2, y3 = m.run(x1), m.run(x2), m.run(x3)
     m.close()
     rng = np.random.default_rng(5)
-    for k, (s, b) in enumerate(C4):
+    for k, (s, b) in enumerate(UNEVEN_MIX):
         V = s[0] * s[1] * s[2]
         idx = np.sort(rng.choice(b, 256, replace=False))
         ref = oracle.fftn_numpy(x1[k][idx].astype(np.complex128))
