@@ -17,6 +17,9 @@ namespace fx {
 #define FX_17P_BOTH(T, TAG, VPW, NT, ALG, ATAG, P)                                                \
     FX_17P(T, -1, VPW, NT, ALG, P, "pipe_17x17x17_" TAG "_fwd_v" #VPW "t" #NT "p" #P "_" ATAG); \
     FX_17P(T, +1, VPW, NT, ALG, P, "pipe_17x17x17_" TAG "_bwd_v" #VPW "t" #NT "p" #P "_" ATAG)
+// (One volume per 320-thread workgroup with two pipelined workgroups per CU, P = 2, x-lines
+// in three parts, 161 VGPRs: 40.6 us at batch 1024 against 32.5 us for the first variant
+// below on the same device, with claims spread over queues; not registered.)
 // Staggered HBM traffic (PipeCfg ST = 1): suffix "_st"; ST = 2, the next block's x-lines
 // issued in three parts: suffix "_st3".  HN = streaming hints (1 loads, 2 stores; the
 // pipelined y pass never hints its 17-element store runs).  Two volumes per 640-thread
