@@ -1,5 +1,5 @@
 // LDS-resident fused kernels for the small geometries: L = 6, 8, 12 and 8x8x16, both
-// precisions and both directions, at several tilings each for the create-time race.
+// precisions and both directions, and L = 24 at complex64, at several tilings each for the create-time race.
 // "pipe_pP" variants are the persistent software-pipelined form (at most P workgroups per
 // CU); "pipe_p0" is the same pass structure with one block per workgroup; "fused_" is the
 // staged form (whole block read into LDS first); "shfl_" keeps an 8^3 volume in one
@@ -92,6 +92,10 @@ void register_fused_small(std::vector<KernelDesc>& r)
     FX_P_BOTH(8, 8, 16, float, "c64", 4, 512, 1, 4);
     FX_PV_BOTH(8, 8, 16, double, "c128", 2, 256, 1, 4, 1, "_r1");
     FX_F_BOTH(8, 8, 16, float, "c64", 4, 512, 1);
+    // L = 24 at complex64: one 115 KB volume per workgroup, 576 lines per pass, staged (the
+    // pipelined form spills at the 168-VGPR cap of 9 waves per CU; at complex128 the volume
+    // exceeds LDS and runs as a pencil).
+    FX_F_BOTH(24, 24, 24, float, "c64", 1, 576, 1);
 }
 
 }  // namespace fx
