@@ -64,8 +64,11 @@ int fft3d_mx_supports(int n0, int n1, int n2, int precision);
 /* Plan a transform of up to `batch` volumes on the current HIP device.  Computes the
  * twiddle / Rader tables in long double, uploads them, and -- when several kernels can
  * serve the geometry -- times each on the device and keeps the fastest (the choice is
- * cached per process, and in the file named by $FFT3D_MX_WISDOM when that is set).
- * $FFT3D_MX_KERNEL=<name> forces a kernel by name.  NULL on failure. */
+ * cached per process, and in the file named by $FFT3D_MX_WISDOM when that is set).  For
+ * batch >= 16 the race also runs at batch / 8, and executes of at most
+ * sqrt(batch * (batch / 8)) volumes use that winner if it differs (plan_info's
+ * kernel_small, small_max).  $FFT3D_MX_KERNEL=<name> forces a kernel by name.  NULL on
+ * failure. */
 fft3d_mx_plan *fft3d_mx_create(int n0, int n1, int n2, int batch, int direction,
                                int precision, unsigned flags);
 
@@ -105,6 +108,8 @@ typedef struct fft3d_mx_plan_info {
     size_t lds_bytes;    /* LDS per workgroup of the main kernel */
     double tune_us;      /* create-time race result of the winner (0 if no race) */
     char kernel[96];     /* kernel name */
+    long small_max;      /* executes of at most this many volumes use kernel_small (0: none) */
+    char kernel_small[96];  /* small-batch kernel name ("" if none) */
 } fft3d_mx_plan_info;
 int fft3d_mx_plan_get_info(const fft3d_mx_plan *plan, fft3d_mx_plan_info *info);
 

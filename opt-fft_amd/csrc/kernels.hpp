@@ -369,14 +369,14 @@ using PlainCfg = HintCfg<C, 0>;
 struct ClaimQueue {
     long q, nqueues, gq, nq;
 };
-FX_HD inline ClaimQueue claim_queue(long w, long grid, long nblocks)
+FX_HD ClaimQueue claim_queue(long w, long grid, long nblocks)
 {
     const long Q = grid < kClaimQueues ? grid : kClaimQueues;
     const long q = w % Q;
     return ClaimQueue{q, Q, (grid - q + Q - 1) / Q, (nblocks - q + Q - 1) / Q};
 }
 // Block of ticket t of queue c; >= nblocks once the queue is exhausted.
-FX_HD inline long claim_queue_block(const ClaimQueue& c, long t)
+FX_HD long claim_queue_block(const ClaimQueue& c, long t)
 {
     return c.q + c.nqueues * (c.gq + t);
 }

@@ -209,6 +209,10 @@ struct fft3d_mx_plan {
     int n0, n1, n2, batch, dir, prec;
     unsigned flags;
     const KernelDesc* k;
+    // Small-batch class: executes of at most small_max volumes launch k_small (raced at an
+    // eighth of the planned batch) when it differs from k; NULL otherwise.
+    const KernelDesc* k_small;
+    long small_max;
     void* bank;
     unsigned* work;     // device: the claim counters `claims` hands out (claims.hpp)
     ClaimSlots claims;
@@ -218,34 +222,42 @@ struct fft3d_mx_plan {
     int cus;            // compute units of the device
 };
 
-// True when a launch of `batch` volumes may draw blocks from a claim counter.  A persistent
-// launch claims only when its blocks number at least twice its grid (claim_counter), and its
-// grid is at least min(CUs, blocks), so fewer than 2 x CUs blocks never claim: such launches
-// skip the counter bookkeeping (and, on several streams, its event).
-static bool may_claim(const fft3d_mx_plan* p, long batch)
+// True when a launch of kernel k over `batch` volumes may draw blocks from claim counters.
+// A persistent launch claims only when its blocks number at least twice its grid
+// (claim_counter), and its grid is at least min(CUs, blocks), so fewer than 2 x CUs blocks
+// never claim: such launches skip the counter bookkeeping (and, on several streams, its
+// event).
+static bool may_claim(const fft3d_mx_plan* p, const KernelDesc* k, long batch)
 {
-    if (p->k->persist <= 0) return false;
-    const long units = p->k->kind == kPencil ? batch * p->n0 : batch;
-    const long nblocks = (units + p->k->vpw - 1) / p->k->vpw;
+    if (k->persist <= 0) return false;
+    const long units = k->kind == kPencil ? batch * p->n0 : batch;
+    const long nblocks = (units + k->vpw - 1) / k->vpw;
     return nblocks >= 2L * p->cus;
+}
+
+// The plan's kernel for a launch of `batch` volumes.
+static const KernelDesc* kernel_for(const fft3d_mx_plan* p, long batch)
+{
+    return p->k_small && batch <= p->small_max ? p->k_small : p->k;
 }
 
 // One launch of the plan's kernel with a claim counter no running launch shares.
 static int plan_launch(fft3d_mx_plan* p, const char* who, const void* in, void* out, long batch,
                        double scale, hipStream_t s)
 {
+    const KernelDesc* k = kernel_for(p, batch);
     const hipError_t e =
-        may_claim(p, batch)
+        may_claim(p, k, batch)
             ? p->claims.run(s, [&](unsigned* w) {
-                  return p->k->launch(in, out, batch, p->bank, scale, w, s);
+                  return k->launch(in, out, batch, p->bank, scale, w, s);
               })
-            : p->k->launch(in, out, batch, p->bank, scale, nullptr, s);
+            : k->launch(in, out, batch, p->bank, scale, nullptr, s);
     if (e == HipClaimRT::busy)
         return fail(FFT3D_MX_EINVAL, "%s: %d launches of this plan were already captured into "
                     "graphs; each keeps its own claim counter and no counter is left", who,
                     ClaimSlots::kSlots);
     if (e != hipSuccess)
-        return fail(FFT3D_MX_EHIP, "%s: %s: %s", who, p->k->name, hipGetErrorString(e));
+        return fail(FFT3D_MX_EHIP, "%s: %s: %s", who, k->name, hipGetErrorString(e));
     return FFT3D_MX_OK;
 }
 
@@ -363,22 +375,38 @@ static fft3d_mx_plan* create_impl(int n0, int n1, int n2, int batch, int directi
     }
     p->claims.attach(p->work);
 
+    p->k_small = nullptr;
+    p->small_max = 0;
     if (!pick) {
         if (cands.size() == 1 || (flags & FFT3D_MX_NO_TUNE) || batch == 0) {
             pick = cands[0];
         } else {
-            const std::string key = wisdom_key(n0, n1, n2, precision, direction, batch, algo);
+            // The race (or the wisdom it left) at the planned batch, and at an eighth of it:
+            // a kernel that fills the device at 512 volumes (one workgroup per volume) can
+            // lose to one that spreads a volume over several workgroups at 64.
             std::lock_guard<std::mutex> g(g_wisdom_mu);
             wisdom_load_locked();
-            auto it = g_wisdom.find(key);
-            if (it != g_wisdom.end())
-                for (const KernelDesc* k : cands)
-                    if (it->second == k->name) pick = k;
-            if (!pick) {
+            auto pick_at = [&](long b, double* tune_us) -> const KernelDesc* {
+                const std::string key = wisdom_key(n0, n1, n2, precision, direction, b, algo);
+                auto it = g_wisdom.find(key);
+                if (it != g_wisdom.end())
+                    for (const KernelDesc* k : cands)
+                        if (it->second == k->name) return k;
                 double best_s = 0.0;
-                pick = race(cands, n0, n1, n2, precision, batch, p->bank, p->work, &best_s);
-                p->tune_us = best_s * 1e6;
-                if (best_s > 0) wisdom_store_locked(key, pick->name);
+                const KernelDesc* k = race(cands, n0, n1, n2, precision, b, p->bank, p->work,
+                                           &best_s);
+                if (tune_us) *tune_us = best_s * 1e6;
+                if (best_s > 0) wisdom_store_locked(key, k->name);
+                return k;
+            };
+            pick = pick_at(batch, &p->tune_us);
+            if (batch >= 16) {
+                const long bs = batch / 8;
+                const KernelDesc* ks = pick_at(bs, nullptr);
+                if (ks != pick) {
+                    p->k_small = ks;
+                    p->small_max = (long)std::sqrt((double)batch * (double)bs);
+                }
             }
         }
     }
@@ -488,6 +516,9 @@ extern "C" int fft3d_mx_plan_get_info(const fft3d_mx_plan* p, fft3d_mx_plan_info
     info->lds_bytes = p->k->lds_bytes;
     info->tune_us = p->tune_us;
     snprintf(info->kernel, sizeof info->kernel, "%s", p->k->name);
+    info->small_max = p->k_small ? p->small_max : 0;
+    snprintf(info->kernel_small, sizeof info->kernel_small, "%s",
+             p->k_small ? p->k_small->name : "");
     return FFT3D_MX_OK;
 }
 

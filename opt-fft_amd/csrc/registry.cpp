@@ -11,6 +11,7 @@ const std::vector<KernelDesc>& registry()
         register_fused_small(v);
         register_fused_17(v);
         register_pencil(v);
+        register_split(v);
         register_mfma(v);
         register_f17(v);
         register_dense(v);

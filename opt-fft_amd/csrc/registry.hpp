@@ -42,6 +42,7 @@ struct KernelDesc {
 void register_fused_small(std::vector<KernelDesc>& r);   // 6, 8, 12, 8x8x16
 void register_fused_17(std::vector<KernelDesc>& r);      // 17
 void register_pencil(std::vector<KernelDesc>& r);        // 24, 36, 24x24x48
+void register_split(std::vector<KernelDesc>& r);         // 24 complex128, single pass
 void register_mfma(std::vector<KernelDesc>& r);          // complex64 DFT-as-matrix
 void register_f17(std::vector<KernelDesc>& r);           // 17, balanced four-wave kernel
 void register_dense(std::vector<KernelDesc>& r);         // dense row-column floor, every geometry

@@ -78,7 +78,8 @@ class PlanInfo(ctypes.Structure):
                 ("precision", ctypes.c_int), ("kind", ctypes.c_int),
                 ("volumes_per_wg", ctypes.c_int), ("threads", ctypes.c_int),
                 ("variant", ctypes.c_int), ("lds_bytes", ctypes.c_size_t),
-                ("tune_us", ctypes.c_double), ("kernel", ctypes.c_char * 96)]
+                ("tune_us", ctypes.c_double), ("kernel", ctypes.c_char * 96),
+                ("small_max", ctypes.c_long), ("kernel_small", ctypes.c_char * 96)]
 
 
 _lib = None
@@ -217,6 +218,12 @@ class Plan:
     @property
     def kernel(self):
         return self.info().kernel.decode()
+
+    @property
+    def kernel_small(self):
+        """(kernel, max volumes) of the small-batch class, or None."""
+        inf = self.info()
+        return (inf.kernel_small.decode(), inf.small_max) if inf.small_max else None
 
     def execute(self, din, dout, batch=None, stream=None):
         """Enqueue on `stream` (hip.Stream, raw handle or None) with device addresses."""

@@ -78,7 +78,7 @@ def test_pipelined_variants_claim_blocks_dynamically(emu):
     seen = 0
     for i in range(n):
         name = emu.emu_kernel_name(i).decode()
-        if not name.startswith(("pipe_", "bal_")) or name.endswith("p0") or "p0_" in name:
+        if not name.startswith(("pipe_", "bal_", "split_")) or name.endswith("p0") or "p0_" in name:
             continue
         shape, prec, d, per_wg = parse(name)
         dt = np.complex128 if prec == 0 else np.complex64

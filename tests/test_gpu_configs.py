@@ -34,7 +34,7 @@ def _persistent_groups():
     f = __graft_entry__.load_package().fft3d
     groups = {}
     for name, shape, prec, d in f.kernels():
-        m = re.search(r"_v(\d+)t\d+p(\d+)", name)
+        m = re.search(r"_v(\d+)t\d+(?:r\d+)?p(\d+)", name)
         if m and int(m.group(2)) > 0:
             vpw, P = int(m.group(1)), int(m.group(2))
             need = 4 * P * CUS * vpw
@@ -192,3 +192,23 @@ def test_dense_floor_agrees_with_the_fast_kernels(gpu, shape, batch):
     want = oracle.fftn_numpy(x)
     assert oracle.rel_l2(yd, want) <= 1e-13
     assert oracle.rel_l2(yd, yf) <= 1e-13
+
+
+# ------------------------------------------------------------------------ batch classes
+@pytest.mark.parametrize("prec", [0, 1], ids=["c128", "c64"])
+def test_plan_batch_classes_24(gpu, prec):
+    """A 24^3 plan raced at batch 512 and at 64 (the small-batch class): executes on both
+    sides of the class boundary, each against numpy.fft."""
+    f = gpu.fft3d
+    dt = np.complex128 if prec == 0 else np.complex64
+    plan = f.Plan((24, 24, 24), 512, f.FORWARD, prec)
+    small = plan.kernel_small
+    print(f"{plan.kernel} small class {small}", flush=True)
+    if small is not None:
+        assert small[1] == int(math.sqrt(512 * 64))
+    for b in (1, 5, 64, 181, 182, 300, 512):
+        x = oracle.make_input((24, 24, 24), b, seed=70 + b, dtype=dt)
+        err = oracle.rel_l2(plan.run(x), oracle.fftn_numpy(x))
+        print(f"  batch {b}: rel_l2 {err:.2e}", flush=True)
+        assert err <= TOL[prec], (b, err)
+    plan.close()
