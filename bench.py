@@ -9,8 +9,9 @@ stream, bracketed by device synchronisations (the contract region); then 5 windo
 steps each, n calibrated so a window lasts >= 20 ms, run back to back, and the reported
 rate (value, ms_per_step, roofline) is the median window's, with the spread of the windows
 under "timing" (short regions catch the clock ramp: the reference's driver insists on
->= 20 ms samples for the same reason).  The other workloads (8^3 x 4096, 36^3 x 128, 24x24x48 x 64 and
-the 17^3 inverse) are timed the same way and reported under "secondary".
+>= 20 ms samples for the same reason).  The other workloads (8^3 x 4096, 36^3 x 128,
+24x24x48 x 64, the 17^3 inverse, and 24^3 x 512, the largest batch of the serving mix's
+24^3 requests) are timed the same way and reported under "secondary".
 
 "precision_variants": complex64 on the uneven batch set 6^3 x 65536, 8^3 x 32768,
 12^3 x 8192, 17^3 x 2048 -- as one persistent mixed launch (FFT, and the DFT-as-matrix
@@ -64,6 +65,8 @@ SECONDARY = [
     dict(name="L36_c128_fwd_b128", shape=(36, 36, 36), batch=128, dist="normal_half",
          direction=-1),
     dict(name="24x24x48_c128_fwd_b64", shape=(24, 24, 48), batch=64, dist="normal_half",
+         direction=-1),
+    dict(name="L24_c128_fwd_b512", shape=(24, 24, 24), batch=512, dist="uniform",
          direction=-1),
 ]
 
