@@ -44,15 +44,20 @@ namespace fx {
 
 void register_fused_small(std::vector<KernelDesc>& r)
 {
-    // L = 6: 36 lines per volume per pass.
+    // L = 6: 36 lines per volume per pass.  Staged blocks of 8 volumes on 288 threads (one
+    // line per lane) beat 16 on 576 by 9% at complex64 and 3% at complex128 (65536 volumes);
+    // at L = 12, complex64, one volume on 192 threads beats the pipelined forms by 15%.
     FX_P_BOTH(6, 6, 6, double, "c128", 16, 576, 1, 4);
     FX_P_BOTH(6, 6, 6, double, "c128", 16, 576, 1, 0);
     FX_P_BOTH(6, 6, 6, double, "c128", 32, 576, 1, 4);
     FX_F_BOTH(6, 6, 6, double, "c128", 16, 576, 1);
+    FX_F_BOTH(6, 6, 6, double, "c128", 8, 288, 1);
     FX_P_BOTH(6, 6, 6, float, "c64", 32, 576, 1, 4);
     FX_PV_BOTH(6, 6, 6, double, "c128", 16, 576, 1, 4, 1, "_r1");
     FX_P_BOTH(6, 6, 6, float, "c64", 16, 576, 1, 0);
     FX_F_BOTH(6, 6, 6, float, "c64", 16, 576, 1);
+    FX_F_BOTH(6, 6, 6, float, "c64", 8, 288, 1);
+    FX_F_BOTH(6, 6, 6, float, "c64", 4, 144, 1);
     // L = 8: 64 lines per volume per pass.
     // 8 volumes x 512 threads first: the fastest from HBM at batch 4096 (the race keeps the
     // earlier-registered variant on near-ties).
@@ -85,6 +90,8 @@ void register_fused_small(std::vector<KernelDesc>& r)
     FX_PV_BOTH(12, 12, 12, float, "c64", 4, 576, 0, 4, 0, "_z0");
     FX_P_BOTH(12, 12, 12, float, "c64", 2, 320, 1, 4);
     FX_F_BOTH(12, 12, 12, float, "c64", 4, 576, 1);
+    FX_F_BOTH(12, 12, 12, float, "c64", 1, 192, 1);
+    FX_F_BOTH(12, 12, 12, float, "c64", 2, 384, 1);
     // 8 x 8 x 16: x lines 128 per volume.
     FX_P_BOTH(8, 8, 16, double, "c128", 2, 256, 1, 4);
     FX_P_BOTH(8, 8, 16, double, "c128", 4, 512, 1, 2);
