@@ -85,6 +85,12 @@ struct MultiPreset<T, 3> {
     using G3 = FusedCfg<17, 17, 17, T, -1, 1, NT, 0, 0, 1>;
     using G4 = FusedCfg<8, 8, 16, T, -1, 4, NT, 1, 0, 1>;
 };
+// The staged task structure (block HBM -> LDS in one grouped read, z and y in LDS, x pass
+// LDS -> HBM with whole-line stores; the per-geometry plans of 6^3, 8^3 and 12^3 complex64
+// race to it) was built on presets 0 and 3 and measured slower in one launch: 213-228 us on
+// the mix below against 164.5 us for preset 0, and alone 69.8 us at 6^3 x 65536 against
+// 50.6 us (the per-geometry staged kernel: 40.0 us at six workgroups per CU; here four, and a
+// staged task computes nothing until its whole block has landed).  Not built.
 // Measured on the complex64 mix 6^3 x 65536, 8^3 x 32768, 12^3 x 8192, 17^3 x 2048 (not
 // built): preset 3 with the items interleaved and nontemporal stores 168.6 us, with
 // nontemporal loads, or loads and stores, 174.0 us, against 168.9 us plain on one device;
