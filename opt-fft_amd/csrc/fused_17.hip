@@ -45,8 +45,8 @@ void register_fused_17(std::vector<KernelDesc>& r)
     FX_17_BOTH(double, "c128", 1, 320, 1, "fold");
     FX_17_BOTH(double, "c128", 1, 320, 0, "rader");
     FX_17_BOTH(double, "c128", 1, 192, 1, "fold");
-    FX_17S_BOTH(float, "c64", 1, 320, 1, "fold", 2, 1, 0, "_st");
     FX_17P_BOTH(float, "c64", 1, 320, 1, "fold", 4);
+    FX_17S_BOTH(float, "c64", 1, 320, 1, "fold", 2, 1, 0, "_st");
     FX_17P_BOTH(float, "c64", 2, 640, 1, "fold", 2);
     FX_17_BOTH(float, "c64", 2, 320, 1, "fold");
     FX_17_BOTH(float, "c64", 1, 320, 1, "fold");

@@ -44,19 +44,22 @@ namespace fx {
 
 void register_fused_small(std::vector<KernelDesc>& r)
 {
+    // The first variant of a geometry and precision is the measured fastest at the
+    // configurations' batches (taken without a race: FFT3D_MX_NO_TUNE, the allocation-free
+    // entry point; and kept by the race on near-ties).
     // L = 6: 36 lines per volume per pass.  Staged blocks of 8 volumes on 288 threads (one
     // line per lane) beat 16 on 576 by 9% at complex64 and 3% at complex128 (65536 volumes);
     // at L = 12, complex64, one volume on 192 threads beats the pipelined forms by 15%.
+    FX_F_BOTH(6, 6, 6, double, "c128", 8, 288, 1);
     FX_P_BOTH(6, 6, 6, double, "c128", 16, 576, 1, 4);
     FX_P_BOTH(6, 6, 6, double, "c128", 16, 576, 1, 0);
     FX_P_BOTH(6, 6, 6, double, "c128", 32, 576, 1, 4);
     FX_F_BOTH(6, 6, 6, double, "c128", 16, 576, 1);
-    FX_F_BOTH(6, 6, 6, double, "c128", 8, 288, 1);
+    FX_F_BOTH(6, 6, 6, float, "c64", 8, 288, 1);
     FX_P_BOTH(6, 6, 6, float, "c64", 32, 576, 1, 4);
     FX_PV_BOTH(6, 6, 6, double, "c128", 16, 576, 1, 4, 1, "_r1");
     FX_P_BOTH(6, 6, 6, float, "c64", 16, 576, 1, 0);
     FX_F_BOTH(6, 6, 6, float, "c64", 16, 576, 1);
-    FX_F_BOTH(6, 6, 6, float, "c64", 8, 288, 1);
     FX_F_BOTH(6, 6, 6, float, "c64", 4, 144, 1);
     // L = 8: 64 lines per volume per pass.
     // 8 volumes x 512 threads first: the fastest from HBM at batch 4096 (the race keeps the
@@ -66,9 +69,9 @@ void register_fused_small(std::vector<KernelDesc>& r)
     FX_P_BOTH(8, 8, 8, double, "c128", 8, 512, 1, 0);
     FX_F_BOTH(8, 8, 8, double, "c128", 4, 256, 1);
     FX_P_BOTH(8, 8, 8, double, "c128", 4, 256, 1, 0);
+    FX_F_BOTH(8, 8, 8, float, "c64", 4, 256, 1);
     FX_P_BOTH(8, 8, 8, float, "c64", 4, 256, 1, 8);
     FX_P_BOTH(8, 8, 8, float, "c64", 8, 512, 1, 4);
-    FX_F_BOTH(8, 8, 8, float, "c64", 4, 256, 1);
     FX_FV_BOTH(8, 8, 8, double, "c128", 8, 512, 1, 1, "_r1");
     FX_FV_BOTH(8, 8, 8, double, "c128", 8, 512, 0, 0, "_z0");
     FX_FV_BOTH(8, 8, 8, float, "c64", 4, 256, 1, 1, "_r1");
@@ -79,18 +82,18 @@ void register_fused_small(std::vector<KernelDesc>& r)
     FX_S_BOTH(double, "c128", 8, 1, "n");
     FX_S_BOTH(float, "c64", 4, 1, "n");
     // L = 12: 144 lines per volume per pass.
+    FX_F_BOTH(12, 12, 12, double, "c128", 1, 192, 1);
     FX_P_BOTH(12, 12, 12, double, "c128", 4, 576, 1, 2);
     FX_P_BOTH(12, 12, 12, double, "c128", 1, 192, 1, 8);
     FX_P_BOTH(12, 12, 12, double, "c128", 4, 576, 1, 0);
-    FX_F_BOTH(12, 12, 12, double, "c128", 1, 192, 1);
     FX_F_BOTH(12, 12, 12, double, "c128", 4, 576, 1);
+    FX_F_BOTH(12, 12, 12, float, "c64", 1, 192, 1);
     FX_P_BOTH(12, 12, 12, float, "c64", 4, 576, 1, 4);
     FX_PV_BOTH(12, 12, 12, double, "c128", 4, 576, 1, 2, 1, "_r1");
     FX_PV_BOTH(12, 12, 12, float, "c64", 4, 576, 1, 4, 1, "_r1");
     FX_PV_BOTH(12, 12, 12, float, "c64", 4, 576, 0, 4, 0, "_z0");
     FX_P_BOTH(12, 12, 12, float, "c64", 2, 320, 1, 4);
     FX_F_BOTH(12, 12, 12, float, "c64", 4, 576, 1);
-    FX_F_BOTH(12, 12, 12, float, "c64", 1, 192, 1);
     FX_F_BOTH(12, 12, 12, float, "c64", 2, 384, 1);
     // 8 x 8 x 16: x lines 128 per volume.
     FX_P_BOTH(8, 8, 16, double, "c128", 2, 256, 1, 4);
