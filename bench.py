@@ -24,6 +24,10 @@ numpy.fft in float64, and for the matrix-core kernels the executed MFMA TFLOP/s 
 "small_batch_latency": batch-1 complex128 executes of every cube geometry, device time back
 to back and host round trip, launched directly and replayed from a captured graph.
 
+"host_abi": the reference's host-pointer ABI (fft3d_create / fft3d_execute) at 17^3 x 1024:
+copies in and out of pageable host memory inside every execute, so the rate is the host
+link's, reported beside the device-resident value, never as it.
+
 "serving": the request server under Poisson arrivals (20,000 requests, the mixed stream of
 opt-fft_amd/serving.py, 64 GB budget, 16 streams, 30 us batching window for the
 continuously batched server): the highest arrival rate whose p99 latency stays
@@ -572,6 +576,35 @@ def small_batch_latency(pkg, reps=300):
     return out
 
 
+def host_abi(pkg, L=17, batch=1024, reps=8):
+    """The reference's host-pointer ABI (fft3d_create / fft3d_execute, bench/geom/fft3d_api.h)
+    on this library at the headline size: each execute copies the batch in from pageable host
+    memory, transforms it and copies it back before returning, as bench/geom/driver.c
+    requires, so its rate is bounded by the host link, not HBM.  Median of `reps` executes
+    after one warm-up, output checked against numpy.fft."""
+    f = pkg.fft3d
+    L_ = f.load()
+    x = make_input((L, L, L), batch, seed=9, dist="uniform")
+    y = np.empty_like(x)
+    h = L_.fft3d_create(L, batch)
+    if not h:
+        return None
+    try:
+        L_.fft3d_execute(h, x.ctypes.data, y.ctypes.data)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            L_.fft3d_execute(h, x.ctypes.data, y.ctypes.data)
+            ts.append(time.perf_counter() - t0)
+    finally:
+        L_.fft3d_destroy(h)
+    t = float(np.median(ts))
+    return {"L": L, "batch": batch, "ms_per_execute": t * 1e3, "transforms_per_s": batch / t,
+            "host_link_GBs_each_way": x.nbytes / t / 1e9,
+            "rel_l2_vs_numpy": _rel_l2(y, _np_dft(x, -1)),
+            "note": "pageable host buffers, H2D + transform + D2H per execute; not `value`"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -627,6 +660,9 @@ def main():
     if latency:
         print(f"# batch-1 latency: {latency}", file=sys.stderr, flush=True)
     serve = None if a.no_serving else serving(pkg)
+    habi = None if a.no_latency else host_abi(pkg)
+    if habi:
+        print(f"# host ABI: {habi}", file=sys.stderr, flush=True)
     V = 17 ** 3
     line = {
         "metric": "batched transforms/s + % HBM roofline at L=8/17/36 fp64; serving p99 latency",
@@ -664,6 +700,7 @@ def main():
         "secondary": secondary,
         "precision_variants": precision,
         "small_batch_latency": latency,
+        "host_abi": habi,
         "serving": serve,
     }
     print(json.dumps(line), flush=True)

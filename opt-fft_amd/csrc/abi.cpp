@@ -50,7 +50,8 @@ extern "C" const char* fft3d_name(void) { return "mi355x_hip"; }
 extern "C" const char* fft3d_description(void)
 {
     return "MI355X gfx950 HIP: LDS-resident fused 3-axis kernels (L<=17, Rader-17), "
-           "two-pass LDS/pencil (L=24,36); H2D+kernel+D2H per execute";
+           "LDS+register single pass or two-pass pencil (L=24), pencil (L=36); "
+           "H2D+kernel+D2H per execute";
 }
 
 extern "C" int fft3d_supports(int L) { return cube_supported(L); }
