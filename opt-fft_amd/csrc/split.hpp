@@ -1,19 +1,21 @@
 // Single-pass kernel for volumes that fit in a CU's LDS and registers together but not in
-// LDS alone: 24^3 complex128 is 221 KB against 160 KB of LDS.  The two-pass pencil moves
-// every element through HBM twice; here each element is read once and written once.
+// LDS alone: 24^3 complex128 is 221 KB and 36^3 complex64 373 KB, against 160 KB of LDS.
+// The two-pass pencil moves every element through HBM twice; here each element is read
+// once and written once.
 //
-// Per volume (one workgroup, one thread per x-line):
+// Per volume (one workgroup, LPT x-lines per thread):
 //   * x-lines (y, z) HBM -> VGPRs (a wave's load instruction reads 64 consecutive elements),
 //     x DFT in registers;
 //   * R rounds, each over XR = N0 / R x-planes: the round's outputs of the x DFT go to LDS
 //     (XR planes, rows padded by PZ), z lines and then y lines are transformed there, and the
 //     y pass stores straight to HBM (a store instruction writes runs of N2 elements).  The
-//     z and y lines run in two in-place stages (ct2_stage1/2), so a thread holds 8 points
-//     of a line rather than all 24;
+//     z and y lines run in two in-place stages (ct2_stage1/2), so a thread holds 8 (or 9)
+//     points of a line rather than all 24 (or 36);
 //   * the next volume's x-lines are loaded into the VGPRs freed by the last round's LDS
 //     write, so they are in flight during that round's z and y passes.
 // During every round but the last, the x-planes of the later rounds stay in VGPRs beside the
-// line being transformed (at 24^3 complex128, R = 2: 12 of 24 points of a line, 48 VGPRs).
+// line being transformed (at 24^3 complex128, R = 2: 12 of 24 points of a line, 48 VGPRs;
+// at 36^3 complex64, R = 3, two lines: 24 of 36 points each, 96 VGPRs, after round 0).
 // Workgroups of the persistent form (P > 0) claim volumes from the launch's claim queues.
 #pragma once
 #include <utility>
@@ -95,6 +97,10 @@ template <>
 struct Ct2<24> {
     static constexpr int A = 3, B = 8;
 };
+template <>
+struct Ct2<36> {
+    static constexpr int A = 4, B = 9;
+};
 
 template <class C>
 struct Split {
@@ -115,11 +121,22 @@ struct Split {
             for (int j = J0; j < J1; ++j) R[i][j] = ld_hint<C::HN & 1>(src + j * C::PLANE + l);
         }
     }
-    static FX_HD void xdft(int tid, Regs& R, const Bank<T>* __restrict__ bk)
+    // x DFT of every line in registers; each line's round-0 x-planes go to LDS as soon as
+    // it is transformed, so with two lines per thread the first line holds only its later
+    // rounds' points while the second is transformed.
+    static FX_HD void xdft_put0(int tid, Regs& R, cx<T>* sm, const Bank<T>* __restrict__ bk)
     {
 #pragma unroll
-        for (int i = 0; i < C::LPT; ++i)
-            if (tid + i * C::NT < C::PLANE) dft<C::N0, C::S, T, C::V17>(R[i], bk);
+        for (int i = 0; i < C::LPT; ++i) {
+            const int l = tid + i * C::NT;
+            if (l < C::PLANE) {
+                dft<C::N0, C::S, T, C::V17>(R[i], bk);
+                const int y = l / C::N2;
+                const int z = l - y * C::N2;
+#pragma unroll
+                for (int xl = 0; xl < C::XR; ++xl) sm[xl * C::XS + y * C::YS + z] = R[i][xl];
+            }
+        }
     }
     // Round RD: x-planes [RD * XR, (RD + 1) * XR) of the x DFT's output to LDS.
     template <int RD>
@@ -177,8 +194,9 @@ struct Split {
     }
 };
 
-// One round that is not the last: x-planes to LDS, z pass, y pass to HBM.  Barriers
-// between the phases and after the y pass (the next round overwrites the LDS).
+// One round that is not the last: x-planes to LDS (round 0's are already there, written by
+// xdft_put0), z pass, y pass to HBM.  Barriers between the phases and after the y pass (the
+// next round overwrites the LDS).
 template <class C, int RD>
 __device__ __forceinline__ void split_round(int tid, long b, typename Split<C>::Regs& R,
                                             cx<typename C::T>* sm,
@@ -187,7 +205,7 @@ __device__ __forceinline__ void split_round(int tid, long b, typename Split<C>::
                                             typename C::T scl)
 {
     using K = Split<C>;
-    K::template put<RD>(tid, R, sm);
+    if constexpr (RD > 0) K::template put<RD>(tid, R, sm);
     __syncthreads();
     K::zpass(tid, sm, bk);
     __syncthreads();
@@ -227,9 +245,9 @@ __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::W
         asm volatile("" : "+v"(tid));
         long claim = 0;
         if (tid == 0) claim = claim_block(work, nvol, b);
-        K::xdft(tid, R, bk);
+        K::xdft_put0(tid, R, sm, bk);
         split_rounds<C>(std::make_integer_sequence<int, C::R - 1>{}, tid, b, R, sm, out, bk, scl);
-        K::template put<C::R - 1>(tid, R, sm);
+        if constexpr (C::R > 1) K::template put<C::R - 1>(tid, R, sm);
         if (tid == 0) s_next = claim;
         __syncthreads();
         const long nb = C::P > 0 ? s_next : nvol;

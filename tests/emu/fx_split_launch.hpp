@@ -37,13 +37,16 @@ hipError_t launch_split(const void* in, void* out, long nvol, const void* bank, 
         nvol, grid, C::P > 0 && nvol >= 2 * grid ? work : nullptr,
         [&](long w, long b) { emu_threads(C::NT, [&](int t) { K::load(t, b, src, R[w][t].r); }); },
         [&](long w, long b, long nb) {
-            emu_threads(C::NT, [&](int t) { K::xdft(t, R[w][t].r, bk); });
+            lds.poison();
+            emu_threads(C::NT, [&](int t) { K::xdft_put0(t, R[w][t].r, lds.data(), bk); });
             auto round = [&](auto rd, bool last) {
                 constexpr int RD = decltype(rd)::value;
-                lds.poison();
                 cx<T>* sm = lds.data();
                 const bool pre = last && C::P > 0 && nb < nvol;
-                emu_threads(C::NT, [&](int t) { K::template put<RD>(t, R[w][t].r, sm); });
+                if constexpr (RD > 0) {
+                    lds.poison();
+                    emu_threads(C::NT, [&](int t) { K::template put<RD>(t, R[w][t].r, sm); });
+                }
                 if (pre)
                     emu_threads(C::NT, [&](int t) { K::template load<0, C::SP1>(t, nb, src, R[w][t].r); });
                 emu_threads(C::NT, [&](int t) { K::zpass(t, sm, bk); });

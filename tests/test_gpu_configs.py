@@ -195,19 +195,20 @@ def test_dense_floor_agrees_with_the_fast_kernels(gpu, shape, batch):
 
 
 # ------------------------------------------------------------------------ batch classes
-@pytest.mark.parametrize("prec", [0, 1], ids=["c128", "c64"])
-def test_plan_batch_classes_24(gpu, prec):
-    """A 24^3 plan raced at batch 512 and at 64 (the small-batch class): executes on both
-    sides of the class boundary, each against numpy.fft."""
+@pytest.mark.parametrize("L,prec", [(24, 0), (24, 1), (36, 1)], ids=["24_c128", "24_c64", "36_c64"])
+def test_plan_batch_classes(gpu, L, prec):
+    """A plan raced at batch 512 and at 64 (the small-batch class), at geometries where the
+    split-resident single pass and the pencil trade places: executes on both sides of the
+    class boundary, each against numpy.fft."""
     f = gpu.fft3d
     dt = np.complex128 if prec == 0 else np.complex64
-    plan = f.Plan((24, 24, 24), 512, f.FORWARD, prec)
+    plan = f.Plan((L, L, L), 512, f.FORWARD, prec)
     small = plan.kernel_small
     print(f"{plan.kernel} small class {small}", flush=True)
     if small is not None:
         assert small[1] == int(math.sqrt(512 * 64))
     for b in (1, 5, 64, 181, 182, 300, 512):
-        x = oracle.make_input((24, 24, 24), b, seed=70 + b, dtype=dt)
+        x = oracle.make_input((L, L, L), b, seed=70 + b, dtype=dt)
         err = oracle.rel_l2(plan.run(x), oracle.fftn_numpy(x))
         print(f"  batch {b}: rel_l2 {err:.2e}", flush=True)
         assert err <= TOL[prec], (b, err)
