@@ -54,9 +54,9 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the headline kernel (pipe_17x17x17_c128_fwd_v2t640p1_fold_st3, the
 # race's pick) from rocprofv3 FETCH_SIZE + WRITE_SIZE, each scaled by the same counter on the
-# runtime's copy of a known size: 39,995.5 KiB x 2 + 78,796.9 KiB
+# runtime's copy of a known size: 39,988.7 KiB x 2 + 78,978.4 KiB
 # (profiles/k_fpipe_17x17x17_c128_fwd_b1024.md).
-TRAFFIC = int((39995.5 * 2 + 78796.859375) * 1024)
+TRAFFIC = int((39988.65625 * 2 + 78978.421875) * 1024)
 TRAFFIC_KERNEL = "pipe_17x17x17_c128_fwd_v2t640p1_fold_st3"
 ROTATE_BYTES = 2 << 30
 
