@@ -43,6 +43,7 @@ import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 
@@ -295,7 +296,8 @@ def _uniform_c64(rng, shape, batch):
     return (rng.uniform(-1, 1, full) + 1j * rng.uniform(-1, 1, full)).astype(np.complex64)
 
 
-# Volumes per block of the matrix-core kernels (standalone and mixed launch alike).
+# Volumes per block of the mixed matrix-core launch; a standalone kernel's is in its name
+# ("_v<VPW>t").
 MFMA_VPW = {(6, 6, 6): 16, (8, 8, 8): 8, (12, 12, 12): 2, (17, 17, 17): 1, (8, 8, 16): 4}
 
 
@@ -305,7 +307,8 @@ def mfma_flops(shape, nvol, kernel=""):
     tile for the compensated split), TM = ceil(2N/16); K = 16 instructions (TK = ceil(2N/16),
     2*16*16*16 flops each), or K = 32 ones (TK = ceil(2N/32), 2*16*16*32 flops each) on the
     axes with 2N > 16 of a "k32" kernel."""
-    vpw = MFMA_VPW[tuple(shape)]
+    m = re.search(r"_v(\d+)t\d+", kernel)
+    vpw = int(m.group(1)) if m else MFMA_VPW[tuple(shape)]
     V = shape[0] * shape[1] * shape[2]
 
     def per(nv):

@@ -1,31 +1,38 @@
 // DFT-as-dense-matrix variant on the matrix cores (complex64, L <= 17).
 //
 // A block of VPW volumes is staged in LDS (complex float).  Each axis pass is, per tile of
-// 16 lines, the real product
-//       [Yr]   [ Wr  -Wi ] [Xr]          W[j][i] = exp(S 2 pi i j i / N)
-//       [Yi] = [ Wi   Wr ] [Xi]
-// of a (2N x 2N) matrix (rows zero-padded to a multiple of 16, columns to a multiple of
-// the instruction's K) with a (2N x 16) operand.  K is 16 (v_mfma_f32_16x16x16*) or 32
-// (gfx950 v_mfma_f32_16x16x32_*, KW = 32, for 2N > 16: a 17-point tile is 3 x 2
-// instructions instead of 3 x 3, but its K pads 34 to 64 instead of 48, and those kernels
-// hold more operand registers; they measured 10-25% slower at N = 12 and 17 and stay in the
-// race as the "k32" variants).  Both operands are split into a high and a
-// low half-precision part (x = xh + xl, xl = x - xh, each rounded to nearest even by the
-// conversion instructions) and three products xh*Wh + xl*Wh + xh*Wl are accumulated in
-// FP32 -- the compensated scheme; the dropped xl*Wl term and the rounding of the low parts
-// are ~2^-18 of the result (bf16) or ~2^-24 (fp16).  SPLIT = 0 uses bf16 parts (~16
-// significant bits, the FP32 exponent range), SPLIT = 1 fp16 parts (~22 bits).  fp16 parts
-// hold only |x| < 65504 and the low part of a small x falls below the normal range, so with
-// SPLIT = 1 every tile of 16 lines is multiplied, before the split, by the power of two that
-// brings its largest magnitude into [0.5, 1) (a wave-wide maximum through DPP lane moves),
-// and the result by the inverse power; both scalings are exact.  The matrix fragments come
-// from the plan-time bank and live in VGPRs for the whole pass.
+// 16 lines, the real product Y = A X of a (2N x 2N) matrix with a (2N x 16) operand whose
+// rows interleave real and imaginary parts (row 2j holds Re x_j, row 2j+1 Im x_j; the same
+// for the outputs), so that
+//       A[2o][2j] = Wr   A[2o][2j+1] = -Wi   A[2o+1][2j] = Wi   A[2o+1][2j+1] = Wr,
+// W[o][j] = exp(S 2 pi i o j / N), rows zero-padded to a multiple of 16 and columns to a
+// multiple of the instruction's K.  K is 16 (v_mfma_f32_16x16x16*) or 32 (gfx950
+// v_mfma_f32_16x16x32_*, KW = 32, for 2N > 16: a 17-point tile is 3 x 2 instructions instead
+// of 3 x 3, but its K pads 34 to 64 instead of 48, and those kernels hold more operand
+// registers; they stay in the race as the "k32" variants).  With the interleaved rows a
+// lane's operand values are whole complex points, one 8-byte LDS read each, at per-lane
+// offsets computed once per pass; the padded positions read point N-1 again (their matrix
+// columns are zero, so any finite value gives the same sums).  Both operands are split into
+// a high and a low half-precision part (x = xh + xl, xl = x - xh, each rounded to nearest
+// even by the conversion instructions) and three products xh*Wh + xl*Wh + xh*Wl are
+// accumulated in FP32 -- the compensated scheme; the dropped xl*Wl term and the rounding of
+// the low parts are ~2^-18 of the result (bf16) or ~2^-24 (fp16).  SPLIT = 0 uses bf16 parts
+// (~16 significant bits, the FP32 exponent range), SPLIT = 1 fp16 parts (~22 bits).  fp16
+// parts hold only |x| < 65504 and the low part of a small x falls below the normal range, so
+// with SPLIT = 1 every line is multiplied, before the split, by the power of two that brings
+// its largest magnitude into [0.5, 1) (a maximum over the four lanes holding the line), and
+// its outputs by the inverse power; both scalings are exact.  The matrix
+// fragments come from the plan-time bank and live in VGPRs for the whole pass.
 //
 // Operand layouts, 64 lanes, Q = K/4 values per lane, q = 0..Q-1:
 //   A: lane l holds A[l % 16][Q (l / 16) + q];  B: B[Q (l / 16) + q][l % 16];
 //   D: D[4 (l / 16) + i][l % 16],  i = 0..3  (both K widths).
+// So in a k-tile lane l holds Q/2 consecutive points of line l % 16, and in an m-tile two
+// consecutive output points.
 #pragma once
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -36,6 +43,32 @@ typedef _Float16 mf_v4h __attribute__((ext_vector_type(4)));
 typedef __bf16 mf_v8b __attribute__((ext_vector_type(8)));
 typedef _Float16 mf_v8h __attribute__((ext_vector_type(8)));
 typedef float mf_v4f __attribute__((ext_vector_type(4)));
+typedef float mf_v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 mf_v2b __attribute__((ext_vector_type(2)));
+typedef _Float16 mf_v2h __attribute__((ext_vector_type(2)));
+typedef unsigned mf_v2u __attribute__((ext_vector_type(2)));
+typedef unsigned mf_v4u __attribute__((ext_vector_type(4)));
+
+// Two values split at once into packed high and low parts (one pair conversion each, the
+// same rounding as the element-wise split): SPLIT 0 bf16, 1 fp16.
+template <int SPLIT>
+__device__ __forceinline__ void split_pair(float x0, float x1, unsigned& hi, unsigned& lo)
+{
+    using H2 = typename std::conditional<SPLIT == 0, mf_v2b, mf_v2h>::type;
+    const mf_v2f x = {x0, x1};
+    const H2 h = __builtin_convertvector(x, H2);
+    const mf_v2f r = x - __builtin_convertvector(h, mf_v2f);
+    hi = __builtin_bit_cast(unsigned, h);
+    lo = __builtin_bit_cast(unsigned, __builtin_convertvector(r, H2));
+}
+
+// Q/2 packed words as the instruction's operand vector.
+template <typename V, int NW>
+__device__ __forceinline__ V pack_words(const unsigned (&w)[NW])
+{
+    if constexpr (NW == 2) return __builtin_bit_cast(V, mf_v2u{w[0], w[1]});
+    else return __builtin_bit_cast(V, mf_v4u{w[0], w[1], w[2], w[3]});
+}
 
 // Split and product for SPLIT (0 bf16, 1 fp16) and the instruction's K width.
 template <int SPLIT, int KW>
@@ -101,26 +134,17 @@ struct MfmaOps<1, 32> {
     }
 };
 
-// max(x, x moved by the DPP control CTRL) on the rows in ROWS (others keep x).
-template <int CTRL, int ROWS>
-__device__ __forceinline__ float dpp_max(float x)
+// Largest of v over the four lanes that hold one line's operand values (l, l ^ 16, l ^ 32,
+// l ^ 48).  v >= 0, so its bit pattern orders like its value.
+__device__ __forceinline__ float line_max(float v)
 {
-    const int y = __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, x), __builtin_bit_cast(int, x),
-                                              CTRL, ROWS, 0xF, false);
-    return fmaxf(x, __builtin_bit_cast(float, y));
-}
-
-// Largest value of v over the wave, uniform: quad, half-row and row butterflies, then the
-// row broadcasts fold rows 0-2 into row 3, whose lane 63 holds the maximum.
-__device__ __forceinline__ float wave_max(float v)
-{
-    v = dpp_max<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
-    v = dpp_max<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
-    v = dpp_max<0x141, 0xF>(v);  // row_half_mirror
-    v = dpp_max<0x140, 0xF>(v);  // row_mirror
-    v = dpp_max<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
-    v = dpp_max<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+    const int lane4 = (int)(threadIdx.x & 63) << 2;
+    unsigned u = __builtin_bit_cast(unsigned, v);
+    unsigned w = (unsigned)__builtin_amdgcn_ds_bpermute(lane4 ^ (16 << 2), (int)u);
+    u = u > w ? u : w;
+    w = (unsigned)__builtin_amdgcn_ds_bpermute(lane4 ^ (32 << 2), (int)u);
+    u = u > w ? u : w;
+    return __builtin_bit_cast(float, u);
 }
 
 // Matrix fragments of one line length N for this lane: hi and lo parts, TM x TK tiles.
@@ -135,15 +159,15 @@ struct MfmaMatrix {
     static constexpr int Q = Ops::Q;
     typename Ops::V hi[TM][TK], lo[TM][TK];
 
-    // A[m][k] of the real block matrix.
+    // A[m][k] of the real block matrix, rows and columns interleaved (re, im) per point.
     __device__ __forceinline__ static float entry(int m, int k, const Bank<float>* __restrict__ bk)
     {
-        if (m >= K2 || k >= K2) return 0.f;
-        const int j = m < N ? m : m - N, i = k < N ? k : k - N;
-        const cx<float> w = bk->w[woff(N) + (j * i) % N];
+        const int o = m >> 1, j = k >> 1;
+        if (o >= N || j >= N) return 0.f;
+        const cx<float> w = bk->w[woff(N) + (o * j) % N];
         const float wr = w.r, wi = S < 0 ? w.i : -w.i;
-        if (m < N) return k < N ? wr : -wi;
-        return k < N ? wi : wr;
+        if ((m & 1) == 0) return (k & 1) == 0 ? wr : -wi;
+        return (k & 1) == 0 ? wi : wr;
     }
     __device__ __forceinline__ void build(int lane, const Bank<float>* __restrict__ bk)
     {
@@ -159,20 +183,63 @@ struct MfmaMatrix {
     }
 };
 
-// One axis pass over nlines lines (line l starts at base(l), N points STRIDE apart), in
-// place in LDS, tiles of 16 lines per wave.
-template <int N, int STRIDE, int S, int SPLIT, int NT, int KW_, typename BaseFn>
-__device__ __forceinline__ void mfma_pass(cx<float>* sm, int nlines, BaseFn base,
-                                          const MfmaMatrix<N, S, SPLIT, KW_>& A, float scl)
+// The distinct line matrices of one geometry, built once and used by every pass: z (N2),
+// and y (N1) and x (N0) only where their lengths differ from those already held.
+template <class C, int SPLIT, int KW>
+struct MfmaMats {
+    using MZ = MfmaMatrix<C::N2, C::S, SPLIT, KW>;
+    using MY = MfmaMatrix<C::N1, C::S, SPLIT, KW>;
+    using MX = MfmaMatrix<C::N0, C::S, SPLIT, KW>;
+    static constexpr bool kOwnY = C::N1 != C::N2;
+    static constexpr bool kOwnX = C::N0 != C::N2 && C::N0 != C::N1;
+    MZ az;
+    MY ay;
+    MX ax;
+    __device__ __forceinline__ void build(int lane, const Bank<float>* __restrict__ bk)
+    {
+        az.build(lane, bk);
+        if constexpr (kOwnY) ay.build(lane, bk);
+        if constexpr (kOwnX) ax.build(lane, bk);
+    }
+    __device__ __forceinline__ const MZ& z() const { return az; }
+    __device__ __forceinline__ const MY& y() const
+    {
+        if constexpr (kOwnY) return ay;
+        else return az;
+    }
+    __device__ __forceinline__ const MX& x() const
+    {
+        if constexpr (C::N0 == C::N2) return az;
+        else if constexpr (C::N0 == C::N1) return ay;
+        else return ax;
+    }
+};
+
+// One axis pass over nlines lines (line l starts at base(l), N points STRIDE apart, in LDS),
+// tiles of 16 lines per wave; output point o of line l goes to store(base(l), l, o, value)
+// (in place in LDS, or to HBM).
+template <int N, int STRIDE, int S, int SPLIT, int NT, int KW_, typename BaseFn, typename StoreFn>
+__device__ __forceinline__ void mfma_pass(const cx<float>* sm, int nlines, BaseFn base,
+                                          const MfmaMatrix<N, S, SPLIT, KW_>& A, float scl,
+                                          StoreFn store)
 {
     using M = MfmaMatrix<N, S, SPLIT, KW_>;
     using Ops = typename M::Ops;
-    constexpr int TM = M::TM, TK = M::TK, KW = M::KW, Q = M::Q;
+    constexpr int TM = M::TM, TK = M::TK, KW = M::KW, Q = M::Q, P = Q / 2;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4;
     const int ntiles = (nlines + 15) / 16;
-    float* smf = (float*)sm;
+    // Point offsets of this lane's operand values (padded positions repeat point N - 1).
+    int goff[TK][P];
+#pragma unroll
+    for (int kt = 0; kt < TK; ++kt)
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const int j = (KW * kt + Q * g) / 2 + p;
+            goff[kt][p] = (j < N ? j : N - 1) * STRIDE;
+        }
     for (int tile = wave; tile < ntiles; tile += NT / 64) {
-        const int line = tile * 16 + lane % 16;
+        const int line = tile * 16 + (lane & 15);
         const bool valid = line < nlines;
         const int b = base(valid ? line : nlines - 1);
         float v[TK][Q];
@@ -180,17 +247,15 @@ __device__ __forceinline__ void mfma_pass(cx<float>* sm, int nlines, BaseFn base
 #pragma unroll
         for (int kt = 0; kt < TK; ++kt)
 #pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const int k = KW * kt + Q * (lane / 16) + q;
-                float x = 0.f;
-                if (k < N) x = smf[2 * (b + k * STRIDE)];
-                else if (k < 2 * N) x = smf[2 * (b + (k - N) * STRIDE) + 1];
-                v[kt][q] = x;
-                mx = fmaxf(mx, fabsf(x));
+            for (int p = 0; p < P; ++p) {
+                const cx<float> x = sm[b + goff[kt][p]];
+                v[kt][2 * p] = x.r;
+                v[kt][2 * p + 1] = x.i;
+                mx = fmaxf(mx, fmaxf(fabsf(x.r), fabsf(x.i)));
             }
         float up = 1.f, down = scl;
         if constexpr (SPLIT == 1) {
-            mx = wave_max(mx);
+            mx = line_max(mx);
             int e = 0;
             if (mx > 0.f) (void)frexpf(mx, &e);
             up = ldexpf(1.f, -e);
@@ -198,9 +263,14 @@ __device__ __forceinline__ void mfma_pass(cx<float>* sm, int nlines, BaseFn base
         }
         typename Ops::V bh[TK], bl[TK];
 #pragma unroll
-        for (int kt = 0; kt < TK; ++kt)
+        for (int kt = 0; kt < TK; ++kt) {
+            unsigned wh[P], wl[P];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) Ops::split(v[kt][q] * up, bh[kt], bl[kt], q);
+            for (int p = 0; p < P; ++p)
+                split_pair<SPLIT>(v[kt][2 * p] * up, v[kt][2 * p + 1] * up, wh[p], wl[p]);
+            bh[kt] = pack_words<typename Ops::V>(wh);
+            bl[kt] = pack_words<typename Ops::V>(wl);
+        }
 #pragma unroll
         for (int mt = 0; mt < TM; ++mt) {
             mf_v4f acc = {0.f, 0.f, 0.f, 0.f};
@@ -212,10 +282,9 @@ __device__ __forceinline__ void mfma_pass(cx<float>* sm, int nlines, BaseFn base
             }
             if (valid) {
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int m = 16 * mt + 4 * (lane / 16) + q;
-                    if (m < N) smf[2 * (b + m * STRIDE)] = acc[q] * down;
-                    else if (m < 2 * N) smf[2 * (b + (m - N) * STRIDE) + 1] = acc[q] * down;
+                for (int p = 0; p < 2; ++p) {
+                    const int o = 8 * mt + 2 * g + p;
+                    if (o < N) store(b, line, o, cx<float>{acc[2 * p] * down, acc[2 * p + 1] * down});
                 }
             }
         }
@@ -227,18 +296,68 @@ struct MfmaCfg : C {
     static constexpr int SPLITV = SPLIT;
 };
 
-// One block of VPW volumes: stage, the three axis passes, store.  CONJ: conjugate on load
-// and on store (the backward transform through forward matrices, for the mixed launch).
-// Ends with a barrier, so a caller may reuse `sm` at once.  KW: instruction K (0 or 16, 32).
-template <class C, int SPLIT, int KW = 0>
+// The three axis passes of block `bid`, staged in LDS: z and y in place; x with XD = 1 from
+// LDS straight to `out` (for a fixed output point the 16 lines of a tile are consecutive in
+// (y, z), so each store instruction writes four runs of 16 elements), with XD = 0 in place,
+// then one coalesced copy of the block to `out`.  CONJ: conjugate on store (the backward
+// transform through forward matrices, for the mixed launch).  Ends with a barrier, so a
+// caller may reuse `sm` at once.
+template <class C, int SPLIT, int KW, int XD>
+__device__ __forceinline__ void mfma_passes(long bid, long nvol, cx<float>* __restrict__ out,
+                                            float scl, bool conj,
+                                            const MfmaMats<C, SPLIT, KW>& A, cx<float>* sm)
+{
+    const int nv = Fused<C>::nvols(bid, nvol);
+    mfma_pass<C::N2, 1, C::S, SPLIT, C::NT>(
+        sm, nv * C::N0 * C::N1,
+        [](int l) {
+            const unsigned u = (unsigned)l, v = u / (C::N0 * C::N1), r = u - v * (C::N0 * C::N1);
+            return (int)(v * C::VS + (r / C::N1) * C::XS + (r % C::N1) * C::YS);
+        },
+        A.z(), 1.f, [sm](int b, int, int o, cx<float> y) { sm[b + o] = y; });
+    __syncthreads();
+    mfma_pass<C::N1, C::YS, C::S, SPLIT, C::NT>(
+        sm, nv * C::N0 * C::N2,
+        [](int l) {
+            const unsigned u = (unsigned)l, v = u / (C::N0 * C::N2), r = u - v * (C::N0 * C::N2);
+            return (int)(v * C::VS + (r / C::N2) * C::XS + (r % C::N2));
+        },
+        A.y(), 1.f, [sm](int b, int, int o, cx<float> y) { sm[b + o * C::YS] = y; });
+    __syncthreads();
+    cx<float>* __restrict__ dst = out + bid * C::VPW * (long)C::V;
+    const float ci = conj ? -1.f : 1.f;
+    auto xbase = [](int l) {
+        const unsigned u = (unsigned)l, v = u / C::PLANE, q = u - v * C::PLANE;
+        return (int)(v * C::VS + (q / C::N2) * C::YS + (q % C::N2));
+    };
+    if constexpr (XD == 1) {
+        mfma_pass<C::N0, C::XS, C::S, SPLIT, C::NT>(
+            sm, nv * C::PLANE, xbase, A.x(), scl, [dst, ci](int, int l, int o, cx<float> y) {
+                const unsigned u = (unsigned)l, v = u / C::PLANE;
+                st_stream(dst + (long)(u + v * (C::V - C::PLANE)) + (long)o * C::PLANE,
+                          cx<float>{y.r, ci * y.i});
+            });
+    } else {
+        mfma_pass<C::N0, C::XS, C::S, SPLIT, C::NT>(
+            sm, nv * C::PLANE, xbase, A.x(), scl,
+            [sm](int b, int, int o, cx<float> y) { sm[b + o * C::XS] = y; });
+        __syncthreads();
+        for (int e = threadIdx.x; e < nv * C::V; e += C::NT) {
+            const cx<float> y = sm[Fused<C>::slot(e)];
+            st_stream(dst + e, cx<float>{y.r, ci * y.i});
+        }
+    }
+    __syncthreads();
+}
+
+// One block of VPW volumes: stage (conjugated on load when CONJ), then the passes.
+template <class C, int SPLIT, int KW, int XD>
 __device__ __forceinline__ void mfma_block(long bid, long nvol, const cx<float>* __restrict__ in,
-                                           cx<float>* __restrict__ out,
-                                           const Bank<float>* __restrict__ bk, float scl,
-                                           bool conj, cx<float>* sm)
+                                           cx<float>* __restrict__ out, float scl, bool conj,
+                                           const MfmaMats<C, SPLIT, KW>& A, cx<float>* sm)
 {
     using K = Fused<C>;
     const int tid = threadIdx.x;
-    const int lane = tid & 63;
     const int nv = K::nvols(bid, nvol);
     K::stage(tid, bid, nvol, in, sm);
     __syncthreads();
@@ -246,62 +365,73 @@ __device__ __forceinline__ void mfma_block(long bid, long nvol, const cx<float>*
         for (int e = tid; e < nv * C::V; e += C::NT) sm[K::slot(e)].i = -sm[K::slot(e)].i;
         __syncthreads();
     }
-    {
-        MfmaMatrix<C::N2, C::S, SPLIT, KW> A;
-        A.build(lane, bk);
-        mfma_pass<C::N2, 1, C::S, SPLIT, C::NT>(
-            sm, nv * C::N0 * C::N1,
-            [](int l) {
-                const int v = l / (C::N0 * C::N1);
-                const int r = l - v * (C::N0 * C::N1);
-                return v * C::VS + (r / C::N1) * C::XS + (r % C::N1) * C::YS;
-            },
-            A, 1.f);
-    }
-    __syncthreads();
-    {
-        MfmaMatrix<C::N1, C::S, SPLIT, KW> A;
-        A.build(lane, bk);
-        mfma_pass<C::N1, C::YS, C::S, SPLIT, C::NT>(
-            sm, nv * C::N0 * C::N2,
-            [](int l) {
-                const int v = l / (C::N0 * C::N2);
-                const int r = l - v * (C::N0 * C::N2);
-                return v * C::VS + (r / C::N2) * C::XS + (r % C::N2);
-            },
-            A, 1.f);
-    }
-    __syncthreads();
-    {
-        MfmaMatrix<C::N0, C::S, SPLIT, KW> A;
-        A.build(lane, bk);
-        mfma_pass<C::N0, C::XS, C::S, SPLIT, C::NT>(
-            sm, nv * C::PLANE,
-            [](int l) {
-                const int v = l / C::PLANE;
-                const int q = l - v * C::PLANE;
-                return v * C::VS + (q / C::N2) * C::YS + (q % C::N2);
-            },
-            A, scl);
-    }
-    __syncthreads();
-    cx<float>* __restrict__ dst = out + bid * C::VPW * (long)C::V;
-    const float ci = conj ? -1.f : 1.f;
-    for (int e = tid; e < nv * C::V; e += C::NT) {
-        const cx<float> y = sm[K::slot(e)];
-        st_stream(dst + e, cx<float>{y.r, ci * y.i});
-    }
-    __syncthreads();
+    mfma_passes<C, SPLIT, KW, XD>(bid, nvol, out, scl, conj, A, sm);
 }
 
-template <class C, int SPLIT, int KW = 0>
+// One block per workgroup (grid = nblocks).
+template <class C, int SPLIT, int KW, int XD>
 __global__ void __launch_bounds__(C::NT) __attribute__((amdgpu_waves_per_eu(C::WPE, 8)))
     k_mfma(const cx<float>* __restrict__ in, cx<float>* __restrict__ out, long nvol,
            const Bank<float>* __restrict__ bk, float scl)
 {
     static_assert(sizeof(typename C::T) == 4, "the matrix-core variant is complex64");
     __shared__ cx<float> sm[C::LDS];
-    mfma_block<C, SPLIT, KW>(blockIdx.x, nvol, in, out, bk, scl, false, sm);
+    MfmaMats<C, SPLIT, KW> A;
+    A.build(threadIdx.x & 63, bk);
+    mfma_block<C, SPLIT, KW, XD>(blockIdx.x, nvol, in, out, scl, false, A, sm);
+}
+
+// Persistent and pipelined: at most P workgroups per CU, each walking blocks
+// blockIdx.x + k gridDim.x (every block costs the same, so no claims) with its matrices
+// built once.  The next block's elements are loaded into registers (E per thread, the
+// coalesced order of the staging copy) while this block's passes run, and written to LDS
+// when they end.
+template <class C, int P>
+struct MfmaPipe {
+    static constexpr int E = (C::VPW * C::V + C::NT - 1) / C::NT;
+    static constexpr int WPE = waves_per_eu(C::LDS * (int)sizeof(cx<float>), C::NT, P);
+};
+
+template <class C, int SPLIT, int KW, int P, int XD>
+__global__ void __launch_bounds__(C::NT)
+    __attribute__((amdgpu_waves_per_eu(MfmaPipe<C, P>::WPE, 8)))
+    k_mfma_pipe(const cx<float>* __restrict__ in, cx<float>* __restrict__ out, long nvol,
+                long nblocks, const Bank<float>* __restrict__ bk, float scl)
+{
+    static_assert(sizeof(typename C::T) == 4, "the matrix-core variant is complex64");
+    using K = Fused<C>;
+    constexpr int E = MfmaPipe<C, P>::E;
+    __shared__ cx<float> sm[C::LDS];
+    const int tid = threadIdx.x;
+    MfmaMats<C, SPLIT, KW> A;
+    A.build(tid & 63, bk);
+    cx<float> pf[E];
+    // Loads past a partial last block repeat its last element (no branch per element).
+    auto load = [&](long b) {
+        const int n = K::nvols(b, nvol) * C::V;
+        const cx<float>* __restrict__ src = in + b * C::VPW * (long)C::V;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+            const int e = tid + k * C::NT;
+            pf[k] = ld_hint<1>(src + (e < n ? e : n - 1));
+        }
+    };
+    long bid = blockIdx.x;
+    load(bid);
+    for (;;) {
+        {
+            const int n = K::nvols(bid, nvol) * C::V;
+#pragma unroll
+            for (int k = 0; k < E; ++k)
+                if (tid + k * C::NT < n) sm[K::slot(tid + k * C::NT)] = pf[k];
+        }
+        __syncthreads();
+        const long nxt = bid + gridDim.x;
+        if (nxt < nblocks) load(nxt);
+        mfma_passes<C, SPLIT, KW, XD>(bid, nvol, out, scl, false, A, sm);
+        if (nxt >= nblocks) break;
+        bid = nxt;
+    }
 }
 
 }  // namespace fx

@@ -25,6 +25,18 @@ struct MultiMfmaPreset {
     }
 };
 
+// One block of geometry G with its matrices built for it; the x pass stores straight to HBM
+// for the short lines and through LDS for 17 (the faster form of each).
+template <class G, int SPLIT>
+__device__ __forceinline__ void run_mfma_task(long bid, long nvol, const cx<float>* in,
+                                              cx<float>* out, const Bank<float>* bk, float scl,
+                                              bool conj, cx<float>* sm)
+{
+    MfmaMats<G, SPLIT, 0> A;
+    A.build(threadIdx.x & 63, bk);
+    mfma_block<G, SPLIT, 0, (G::N0 > 12 ? 0 : 1)>(bid, nvol, in, out, scl, conj, A, sm);
+}
+
 // One task per workgroup (grid = ntasks; the hardware dispatcher hands tasks, heaviest
 // first, to whichever CU frees up).  A persistent loop over tasks would keep the matrix
 // fragments of several geometries live across iterations and spill.
@@ -46,11 +58,11 @@ __global__ void __launch_bounds__(MultiMfmaPreset::NT)
     const bool cj = ip->conj != 0;
     const float scl = ip->scale;
     switch (ip->code) {  // uniform over the workgroup
-    case 0: mfma_block<P::G0, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
-    case 1: mfma_block<P::G1, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
-    case 2: mfma_block<P::G2, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
-    case 3: mfma_block<P::G3, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
-    default: mfma_block<P::G4, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
+    case 0: run_mfma_task<P::G0, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
+    case 1: run_mfma_task<P::G1, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
+    case 2: run_mfma_task<P::G2, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
+    case 3: run_mfma_task<P::G3, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
+    default: run_mfma_task<P::G4, SPLIT>(task.y, nvol, in, out, bk, scl, cj, sm); break;
     }
 }
 

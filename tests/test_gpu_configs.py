@@ -2,7 +2,8 @@
 
 * every persistent kernel variant (forced by name) at a batch of at least four times its
   largest resident grid, so each workgroup walks several claimed blocks with the next
-  block's lines prefetched, against numpy.fft on the whole batch;
+  block's lines prefetched (the matrix-core forms: several blocks at a fixed stride with
+  their matrices built once), against numpy.fft on the whole batch;
 * the uneven complex64 mix in ONE persistent launch (6^3 x 65536, 8^3 x 32768,
   12^3 x 8192, 17^3 x 2048) as the FFT and as both matrix-core splits: 256 sampled volumes
   per item against numpy.fft in float64, Parseval and linearity over the full batch;
@@ -26,6 +27,14 @@ pytestmark = pytest.mark.gpu
 
 CUS = 256
 TOL = {0: 1e-13, 1: 2e-6}
+MFMA_TOL = {"bf16x3": 2e-5, "fp16x3": 6e-7}   # the matrix-core variants' stated accuracy
+
+
+def _tol(name, prec):
+    for k, v in MFMA_TOL.items():
+        if name.startswith("mfma_") and name.endswith(k):
+            return v
+    return TOL[prec]
 
 
 def _persistent_groups():
@@ -67,7 +76,8 @@ def test_persistent_variants_walk_many_blocks(gpu, key):
         plan.close()
         err = oracle.rel_l2(y, want)
         print(f"rel_l2={err:.2e}", flush=True)
-        assert err <= TOL[prec], (name, err)
+        assert np.all(np.isfinite(y)), name
+        assert err <= _tol(name, prec), (name, err)
 
 
 # --------------------------------------------------------------------------- uneven mix
