@@ -7,7 +7,9 @@
 
 namespace fx {
 
-// Tiles of the standalone matrix-core kernels (mfma.hip), one workgroup size.
+// Tiles of the matrix-core kernels (mfma.hpp), one workgroup size.  The LDS array is sized
+// for 17^3, so every task runs at four workgroups per CU; halving the 6^3 and 8^3 tiles (the
+// standalone kernels' best) only adds tasks: 200.8 against 192.9 us for the bf16x3 mix.
 struct MultiMfmaPreset {
     static constexpr int NT = 256;
     using G0 = FusedCfg<6, 6, 6, float, -1, 16, NT, 1, 0, 0>;
