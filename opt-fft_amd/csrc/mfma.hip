@@ -6,7 +6,8 @@
 // Name: mfma_<shape>_c64_<dir>_v<VPW>t<NT>[k32][p<P>][d]_<split>x3
 //   k32  the gfx950 K = 32 instructions on the axes with 2N > 16 (otherwise K = 16);
 //   p<P> persistent and pipelined, at most P workgroups per CU (otherwise one block each);
-//   d    the x pass stores straight to HBM (otherwise through LDS and one coalesced copy).
+//   d    the x pass stores straight to HBM (otherwise through LDS and one coalesced copy);
+//   fold 17-point lines by the conjugate-pair fold on K = 32 instructions.
 #include "mfma.hpp"
 
 namespace fx {
@@ -32,9 +33,10 @@ void register_mfma(std::vector<KernelDesc>& r)
     FX_MG4(6, 6, 6, 8, 256, 1, 0, 0, 1, "d");
     FX_MG4(8, 8, 8, 4, 256, 1, 0, 0, 1, "d");
     FX_MG4(12, 12, 12, 2, 256, 1, 0, 4, 1, "p4d");
-    FX_MG4(17, 17, 17, 1, 256, 0, 0, 0, 0, "");
+    FX_MG4(17, 17, 17, 1, 256, 0, kFold17, 3, 0, "p3fold");
     FX_MG4(8, 8, 16, 4, 256, 1, 0, 0, 0, "");
     // One block per workgroup.
+    FX_MG4(17, 17, 17, 1, 256, 0, 0, 0, 0, "");
     FX_MG4(6, 6, 6, 16, 256, 1, 0, 0, 0, "");
     FX_MG4(6, 6, 6, 8, 256, 1, 0, 0, 0, "");
     FX_MG4(8, 8, 8, 8, 256, 1, 0, 0, 0, "");
@@ -46,6 +48,10 @@ void register_mfma(std::vector<KernelDesc>& r)
     // K = 32 on the long axes.
     FX_MG4(12, 12, 12, 2, 256, 1, 32, 0, 1, "k32d");
     FX_MG4(17, 17, 17, 1, 256, 0, 32, 0, 0, "k32");
+    FX_MG4(8, 8, 16, 4, 256, 1, 32, 0, 1, "k32d");
+    // 17-point fold.
+    FX_MG4(17, 17, 17, 1, 256, 0, kFold17, 0, 0, "fold");
+    FX_MG4(17, 17, 17, 1, 256, 0, kFold17, 3, 1, "p3dfold");
     // Persistent and pipelined.
     FX_MG4(6, 6, 6, 8, 256, 1, 0, 8, 1, "p8d");
     FX_MG4(8, 8, 8, 4, 256, 1, 0, 8, 1, "p8d");

@@ -183,6 +183,46 @@ struct MfmaMatrix {
     }
 };
 
+// 17-point lines by the conjugate-pair fold (KW = kFold17 in the kernel templates; 17^3
+// only).  With a_j = x_j + x_{17-j}, b_j = x_j - x_{17-j} (j = 1..8), c = cos(2 pi jk/17),
+// s = S sin(2 pi jk/17):
+//     X_k = x_0 + sum_j (a_j c + i b_j s),  X_{17-k} = x_0 + sum_j (a_j c - i b_j s),  k = 1..8,
+//     X_0 = x_0 + sum_j a_j.
+// Both sums are one K = 32 product each, [a; b] (32 rows, re/im interleaved) by a 16 x 32
+// matrix whose 16 rows are X_1..X_8 (P) or X_16..X_9 (M) less x_0, so a tile of 16 lines is
+// 2 x 3 v_mfma_f32_16x16x32 instructions where the dense 34 x 34 form needs 3 x 3 x 3 of
+// K = 16 (or 3 x 2 x 3 of K = 32 with K padded to 64).  Lane group g = l / 16 holds rows
+// 8g..8g+7 of [a; b]: a_{1..4}, a_{5..8}, b_{1..4}, b_{5..8}.
+constexpr int kFold17 = 1;
+
+template <int S, int SPLIT>
+struct Fold17Matrix {
+    using Ops = MfmaOps<SPLIT, 32>;
+    typename Ops::V ph, pl, mh, ml;   // P and M, high and low parts
+
+    // Row m (output point m/2 + 1, re/im), column c (a or b of point (c % 16)/2 + 1, re/im).
+    __device__ __forceinline__ static float entry(int m, int c, bool minus,
+                                                  const Bank<float>* __restrict__ bk)
+    {
+        const int k = m / 2 + 1, j = (c % 16) / 2 + 1;
+        const cx<float> w = bk->w[woff(17) + (j * k) % 17];   // exp(-2 pi i jk/17)
+        const float cs = w.r, sn = (S < 0 ? w.i : -w.i) * (minus ? -1.f : 1.f);   // S sin
+        const bool re_out = (m & 1) == 0, re_in = (c & 1) == 0;
+        if (c < 16) return re_out == re_in ? cs : 0.f;          // a_j c
+        if (re_out) return re_in ? 0.f : -sn;                    // Re(i b s) = -Im(b) s
+        return re_in ? sn : 0.f;                                 // Im(i b s) =  Re(b) s
+    }
+    __device__ __forceinline__ void build(int lane, const Bank<float>* __restrict__ bk)
+    {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int m = lane % 16, c = 8 * (lane / 16) + q;
+            Ops::split(entry(m, c, false, bk), ph, pl, q);
+            Ops::split(entry(m, c, true, bk), mh, ml, q);
+        }
+    }
+};
+
 // The distinct line matrices of one geometry, built once and used by every pass: z (N2),
 // and y (N1) and x (N0) only where their lengths differ from those already held.
 template <class C, int SPLIT, int KW>
@@ -213,6 +253,20 @@ struct MfmaMats {
         else if constexpr (C::N0 == C::N1) return ay;
         else return ax;
     }
+};
+
+template <class C, int SPLIT>
+struct MfmaMats<C, SPLIT, kFold17> {
+    static_assert(C::N0 == 17 && C::N1 == 17 && C::N2 == 17, "the fold form is for 17^3");
+    using MF = Fold17Matrix<C::S, SPLIT>;
+    MF a;
+    __device__ __forceinline__ void build(int lane, const Bank<float>* __restrict__ bk)
+    {
+        a.build(lane, bk);
+    }
+    __device__ __forceinline__ const MF& z() const { return a; }
+    __device__ __forceinline__ const MF& y() const { return a; }
+    __device__ __forceinline__ const MF& x() const { return a; }
 };
 
 // One axis pass over nlines lines (line l starts at base(l), N points STRIDE apart, in LDS),
@@ -287,6 +341,75 @@ __device__ __forceinline__ void mfma_pass(const cx<float>* sm, int nlines, BaseF
                     if (o < N) store(b, line, o, cx<float>{acc[2 * p] * down, acc[2 * p + 1] * down});
                 }
             }
+        }
+    }
+}
+
+// The fold form of the pass above for 17-point lines (same interface).
+template <int N, int STRIDE, int S, int SPLIT, int NT, typename BaseFn, typename StoreFn>
+__device__ __forceinline__ void mfma_pass(const cx<float>* sm, int nlines, BaseFn base,
+                                          const Fold17Matrix<S, SPLIT>& A, float scl,
+                                          StoreFn store)
+{
+    static_assert(N == 17, "fold form");
+    using Ops = MfmaOps<SPLIT, 32>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4;
+    const int ntiles = (nlines + 15) / 16;
+    // This lane's points j = j0..j0+3 and their partners 17 - j; a (g < 2) or b (g >= 2).
+    const int j0 = 4 * (g & 1) + 1;
+    const float sg = g < 2 ? 1.f : -1.f;
+    for (int tile = wave; tile < ntiles; tile += NT / 64) {
+        const int line = tile * 16 + (lane & 15);
+        const bool valid = line < nlines;
+        const int b = base(valid ? line : nlines - 1);
+        const cx<float> x0 = sm[b];
+        float v[8];
+        float mx = 0.f, sr = 0.f, si = 0.f;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const cx<float> u = sm[b + (j0 + p) * STRIDE];
+            const cx<float> w = sm[b + (17 - j0 - p) * STRIDE];
+            v[2 * p] = fmaf(sg, w.r, u.r);
+            v[2 * p + 1] = fmaf(sg, w.i, u.i);
+            sr += v[2 * p];
+            si += v[2 * p + 1];
+            mx = fmaxf(mx, fmaxf(fabsf(v[2 * p]), fabsf(v[2 * p + 1])));
+        }
+        // X_0: the a sums of groups 0 and 1 (groups 2 and 3 exchange b sums, unused).
+        sr += __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(
+                  ((lane ^ 16) << 2), __builtin_bit_cast(int, sr)));
+        si += __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(
+                  ((lane ^ 16) << 2), __builtin_bit_cast(int, si)));
+        float up = 1.f, down = scl;
+        if constexpr (SPLIT == 1) {
+            mx = line_max(mx);
+            int e = 0;
+            if (mx > 0.f) (void)frexpf(mx, &e);
+            up = ldexpf(1.f, -e);
+            down = ldexpf(scl, e);
+        }
+        unsigned wh[4], wl[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) split_pair<SPLIT>(v[2 * p] * up, v[2 * p + 1] * up, wh[p], wl[p]);
+        const typename Ops::V bh = pack_words<typename Ops::V>(wh);
+        const typename Ops::V bl = pack_words<typename Ops::V>(wl);
+        mf_v4f accp = {0.f, 0.f, 0.f, 0.f}, accm = {0.f, 0.f, 0.f, 0.f};
+        accp = Ops::mma(A.ph, bh, accp);
+        accm = Ops::mma(A.mh, bh, accm);
+        accp = Ops::mma(A.ph, bl, accp);
+        accm = Ops::mma(A.mh, bl, accm);
+        accp = Ops::mma(A.pl, bh, accp);
+        accm = Ops::mma(A.ml, bh, accm);
+        if (valid) {
+            const float xr = x0.r * scl, xi = x0.i * scl;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int k = 2 * g + 1 + p;
+                store(b, line, k, cx<float>{fmaf(accp[2 * p], down, xr), fmaf(accp[2 * p + 1], down, xi)});
+                store(b, line, 17 - k, cx<float>{fmaf(accm[2 * p], down, xr), fmaf(accm[2 * p + 1], down, xi)});
+            }
+            if (g == 0) store(b, line, 0, cx<float>{(x0.r + sr) * scl, (x0.i + si) * scl});
         }
     }
 }

@@ -57,6 +57,25 @@ def test_mfma_accuracy_at_any_input_scale(gpu, split, scale, dc):
         assert err <= BARS[split], (shape, scale, dc, err)
 
 
+def test_every_matrix_core_kernel_forced_by_name(gpu):
+    """Every registered matrix-core form (tile sizes, K widths, the 17-point fold, the x pass
+    through LDS or straight to HBM, persistent and pipelined), forced by name, both
+    directions, at a batch that leaves a partial last block, against the oracle."""
+    f = gpu.fft3d
+    names = [(n, sh, d) for n, sh, prec, d in f.kernels() if n.startswith("mfma_")]
+    assert any("fold" in n for n, _, _ in names)
+    for name, shape, d in names:
+        split = "bf16" if name.endswith("bf16x3") else "fp16"
+        x = oracle.make_input(shape, 37, seed=23, dtype=np.complex64)
+        plan = f.Plan(shape, 37, d, f.C64, kernel=name)
+        y = plan.run(x)
+        plan.close()
+        err = oracle.rel_l2(y, oracle.fftn_numpy(x.astype(np.complex128), sign=d))
+        print(f"{name} B=37 rel_l2={err:.2e}", flush=True)
+        assert np.all(np.isfinite(y)), name
+        assert err <= BARS[split], (name, err)
+
+
 def test_mfma_normalized_round_trip(gpu):
     f = gpu.fft3d
     x = oracle.make_input((17, 17, 17), 8, seed=3, dtype=np.complex64)
