@@ -277,6 +277,7 @@ __device__ __forceinline__ void mfma_pass(const cx<float>* sm, int nlines, BaseF
                                           const MfmaMatrix<N, S, SPLIT, KW_>& A, float scl,
                                           StoreFn store)
 {
+    static_assert(NT % 64 == 0, "whole waves: every lane of a tile takes part in the lane exchanges");
     using M = MfmaMatrix<N, S, SPLIT, KW_>;
     using Ops = typename M::Ops;
     constexpr int TM = M::TM, TK = M::TK, KW = M::KW, Q = M::Q, P = Q / 2;
@@ -352,6 +353,7 @@ __device__ __forceinline__ void mfma_pass(const cx<float>* sm, int nlines, BaseF
                                           StoreFn store)
 {
     static_assert(N == 17, "fold form");
+    static_assert(NT % 64 == 0, "whole waves: every lane of a tile takes part in the lane exchanges");
     using Ops = MfmaOps<SPLIT, 32>;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4;
