@@ -20,8 +20,8 @@
 // (~16 significant bits, the FP32 exponent range), SPLIT = 1 fp16 parts (~22 bits).  fp16
 // parts hold only |x| < 65504 and the low part of a small x falls below the normal range, so
 // with SPLIT = 1 every line is multiplied, before the split, by the power of two that brings
-// its largest magnitude into [0.5, 1) (a maximum over the four lanes holding the line), and
-// its outputs by the inverse power; both scalings are exact.  The matrix
+// its largest magnitude into [0.5, 1) (a maximum over the four lanes holding the line,
+// through two lane-swap moves), and its outputs by the inverse power; both scalings are exact.  The matrix
 // fragments come from the plan-time bank and live in VGPRs for the whole pass.
 //
 // Operand layouts, 64 lanes, Q = K/4 values per lane, q = 0..Q-1:
@@ -134,15 +134,27 @@ struct MfmaOps<1, 32> {
     }
 };
 
+// v and its value in lane l ^ 16 (gfx950 v_permlane16_swap: rows 0 <-> 1 and 2 <-> 3) or
+// l ^ 32 (v_permlane32_swap: rows 0, 1 <-> 2, 3); VALU moves, no LDS round trip.
+__device__ __forceinline__ unsigned xor16(unsigned v)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return r[0] ^ r[1] ^ v;
+}
+__device__ __forceinline__ unsigned xor32(unsigned v)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return r[0] ^ r[1] ^ v;
+}
+
 // Largest of v over the four lanes that hold one line's operand values (l, l ^ 16, l ^ 32,
 // l ^ 48).  v >= 0, so its bit pattern orders like its value.
 __device__ __forceinline__ float line_max(float v)
 {
-    const int lane4 = (int)(threadIdx.x & 63) << 2;
     unsigned u = __builtin_bit_cast(unsigned, v);
-    unsigned w = (unsigned)__builtin_amdgcn_ds_bpermute(lane4 ^ (16 << 2), (int)u);
+    unsigned w = xor16(u);
     u = u > w ? u : w;
-    w = (unsigned)__builtin_amdgcn_ds_bpermute(lane4 ^ (32 << 2), (int)u);
+    w = xor32(u);
     u = u > w ? u : w;
     return __builtin_bit_cast(float, u);
 }
@@ -379,10 +391,8 @@ __device__ __forceinline__ void mfma_pass(const cx<float>* sm, int nlines, BaseF
             mx = fmaxf(mx, fmaxf(fabsf(v[2 * p]), fabsf(v[2 * p + 1])));
         }
         // X_0: the a sums of groups 0 and 1 (groups 2 and 3 exchange b sums, unused).
-        sr += __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(
-                  ((lane ^ 16) << 2), __builtin_bit_cast(int, sr)));
-        si += __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(
-                  ((lane ^ 16) << 2), __builtin_bit_cast(int, si)));
+        sr += __builtin_bit_cast(float, xor16(__builtin_bit_cast(unsigned, sr)));
+        si += __builtin_bit_cast(float, xor16(__builtin_bit_cast(unsigned, si)));
         float up = 1.f, down = scl;
         if constexpr (SPLIT == 1) {
             mx = line_max(mx);
