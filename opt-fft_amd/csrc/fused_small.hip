@@ -37,10 +37,10 @@ namespace fx {
     FX_PV(N0, N1, N2, T, +1, VPW, NT, PZ, P, ALG,                                                 \
           "pipe_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_v" #VPW "t" #NT "p" #P SUF)
 
-#define FX_S(T, S, W, PERM, NAME) r.push_back(shfl8_desc<Shfl8Cfg<T, S, W, PERM>>(NAME))
-#define FX_S_BOTH(T, TAG, W, PERM, ORD)                                            \
-    FX_S(T, -1, W, PERM, "shfl_8x8x8_" TAG "_fwd_w" #W ORD);                       \
-    FX_S(T, +1, W, PERM, "shfl_8x8x8_" TAG "_bwd_w" #W ORD)
+#define FX_S(T, S, W, PERM, X, NAME) r.push_back(shfl8_desc<Shfl8Cfg<T, S, W, PERM, X>>(NAME))
+#define FX_S_BOTH(T, TAG, W, PERM, ORD, X, XT)                                       \
+    FX_S(T, -1, W, PERM, X, "shfl_8x8x8_" TAG "_fwd_w" #W ORD XT);                   \
+    FX_S(T, +1, W, PERM, X, "shfl_8x8x8_" TAG "_bwd_w" #W ORD XT)
 
 void register_fused_small(std::vector<KernelDesc>& r)
 {
@@ -77,10 +77,14 @@ void register_fused_small(std::vector<KernelDesc>& r)
     FX_FV_BOTH(8, 8, 8, float, "c64", 4, 256, 1, 1, "_r1");
     // Register-resident, shuffles for y and z: W waves (volumes) per workgroup; "n" stores
     // in natural lane order after one more shuffle, "r" from the bit-reversed lanes.
-    FX_S_BOTH(double, "c128", 4, 1, "n");
-    FX_S_BOTH(double, "c128", 4, 0, "r");
-    FX_S_BOTH(double, "c128", 8, 1, "n");
-    FX_S_BOTH(float, "c64", 4, 1, "n");
+    FX_S_BOTH(double, "c128", 4, 1, "n", 0, "");
+    FX_S_BOTH(double, "c128", 4, 0, "r", 0, "");
+    FX_S_BOTH(double, "c128", 8, 1, "n", 0, "");
+    FX_S_BOTH(float, "c64", 4, 1, "n", 0, "");
+    // The same with DPP and permlane-swap moves for the exchanges instead of ds_bpermute
+    // (X = 1): 15.5 against 15.0 us at 4096 volumes (the LDS kernel: 10.9 us); the
+    // exchanges do not bound this form, its radix-2 butterflies on every lane do.
+    FX_S_BOTH(double, "c128", 4, 0, "r", 1, "_dpp");
     // L = 12: 144 lines per volume per pass.
     FX_F_BOTH(12, 12, 12, double, "c128", 1, 192, 1);
     FX_P_BOTH(12, 12, 12, double, "c128", 4, 576, 1, 2);

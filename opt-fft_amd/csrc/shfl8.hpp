@@ -6,7 +6,7 @@
 // contiguous 1 KiB run at complex128) and runs the 8-point DFT in registers.  The z axis
 // lives in lane bits 0-2 and the y axis in bits 3-5, so each is an 8-point DFT across lanes:
 // three radix-2 decimation-in-frequency stages, the stage on lane bit m pairing lane l with
-// l ^ m through a wavefront shuffle,
+// l ^ m through a lane exchange (X in Shfl8Cfg),
 //       lane bit clear:  a + b          lane bit set:  (b - a) * W_{2h}^(j mod h)
 // (a this lane's value, b the partner's, h = the stage's half size, j the lane's index
 // along the axis), which leaves frequency bitrev3(j) on lane j.  PERM = 1 then permutes the
@@ -24,10 +24,14 @@
 
 namespace fx {
 
-template <typename T_, int S_, int W_, int PERM_>
+// X: how a stage reaches its partner lane, 0 through ds_bpermute (the LDS crossbar), 1 with
+// VALU moves: DPP within a row of 16 lanes (l ^ 1, l ^ 2: quad permutations; l ^ 4: a
+// half-row mirror then a quad reversal; l ^ 8: a row rotation by 8) and gfx950's
+// v_permlane16_swap / v_permlane32_swap for l ^ 16 and l ^ 32.
+template <typename T_, int S_, int W_, int PERM_, int X_ = 1>
 struct Shfl8Cfg {
     using T = T_;
-    static constexpr int S = S_, W = W_, PERM = PERM_;
+    static constexpr int S = S_, W = W_, PERM = PERM_, X = X_;
     static constexpr int NT = 64 * W;
 };
 
@@ -87,6 +91,58 @@ __device__ __forceinline__ cx<T> shfl_xor_cx(cx<T> v, int m)
 {
     return cx<T>{__shfl_xor(v.r, m), __shfl_xor(v.i, m)};
 }
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp_move(unsigned v)
+{
+    return (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
+}
+// The value of lane l ^ M, with VALU moves (the permlane swaps return this lane's value and
+// its partner's in some order, so their XOR with this lane's value is the partner's).
+template <int M>
+__device__ __forceinline__ unsigned lane_xor(unsigned v)
+{
+    if constexpr (M == 32) {
+        const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return r[0] ^ r[1] ^ v;
+    } else if constexpr (M == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return r[0] ^ r[1] ^ v;
+    } else if constexpr (M == 8) {
+        return dpp_move<0x128>(v);              // row_ror:8
+    } else if constexpr (M == 4) {
+        return dpp_move<0x1B>(dpp_move<0x141>(v));  // row_half_mirror, quad_perm [3,2,1,0]
+    } else if constexpr (M == 2) {
+        return dpp_move<0x4E>(v);               // quad_perm [2,3,0,1]
+    } else {
+        static_assert(M == 1, "lane bit");
+        return dpp_move<0xB1>(v);               // quad_perm [1,0,3,2]
+    }
+}
+template <int M, typename T>
+__device__ __forceinline__ cx<T> lane_xor_cx(cx<T> v)
+{
+    constexpr int NW = (int)(sizeof(cx<T>) / 4);
+    struct Words {
+        unsigned w[NW];
+    };
+    Words a = __builtin_bit_cast(Words, v);
+#pragma unroll
+    for (int i = 0; i < NW; ++i) a.w[i] = lane_xor<M>(a.w[i]);
+    return __builtin_bit_cast(cx<T>, a);
+}
+// One radix-2 stage on lane bit M (axis bits from BASE): partner values, then the butterfly.
+template <class C, int M, int BASE>
+__device__ __forceinline__ void shfl_stage(int lane, typename Shfl8<C>::Regs r,
+                                           typename Shfl8<C>::Regs p,
+                                           const Bank<typename C::T>* __restrict__ bk)
+{
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        if constexpr (C::X == 1) p[x] = lane_xor_cx<M>(r[x]);
+        else p[x] = shfl_xor_cx(r[x], M);
+    }
+    Shfl8<C>::stage(lane, M, BASE, r, p, bk);
+}
 template <typename T>
 __device__ __forceinline__ cx<T> shfl_cx(cx<T> v, int src)
 {
@@ -107,13 +163,12 @@ __global__ void __launch_bounds__(C::NT)
     if (v >= nvol) return;  // whole waves exit: no barrier below
     typename K::Regs r, p;
     K::load_x(lane, in + v * 512, r, bk);
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        const int m = s < 3 ? (4 >> s) : (32 >> (s - 3));
-#pragma unroll
-        for (int x = 0; x < 8; ++x) p[x] = shfl_xor_cx(r[x], m);
-        K::stage(lane, m, s < 3 ? 0 : 3, r, p, bk);
-    }
+    shfl_stage<C, 4, 0>(lane, r, p, bk);
+    shfl_stage<C, 2, 0>(lane, r, p, bk);
+    shfl_stage<C, 1, 0>(lane, r, p, bk);
+    shfl_stage<C, 32, 3>(lane, r, p, bk);
+    shfl_stage<C, 16, 3>(lane, r, p, bk);
+    shfl_stage<C, 8, 3>(lane, r, p, bk);
     if (C::PERM) {
         const int src = K::natural_src(lane);
 #pragma unroll
