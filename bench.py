@@ -306,8 +306,8 @@ def mfma_flops(shape, nvol, kernel=""):
     and per axis pass of length N, ceil(lines/16) tiles of TM x TK x 3 products (three per
     tile for the compensated split), TM = ceil(2N/16); K = 16 instructions (TK = ceil(2N/16),
     2*16*16*16 flops each), or K = 32 ones (TK = ceil(2N/32), 2*16*16*32 flops each) on the
-    axes with 2N > 16 of a "k32" kernel; the 17-point tiles of a "fold" kernel and of the
-    mixed launch are 2 x 3 K = 32 products."""
+    axes with 2N > 16 of a "k32" kernel and of the mixed launch; the 17-point tiles of a
+    "fold" kernel and of the mixed launch are 2 x 3 K = 32 products."""
     m = re.search(r"_v(\d+)t\d+", kernel)
     vpw = int(m.group(1)) if m else MFMA_VPW[tuple(shape)]
     V = shape[0] * shape[1] * shape[2]
@@ -318,7 +318,7 @@ def mfma_flops(shape, nvol, kernel=""):
             if n == 17 and ("fold" in kernel or kernel.startswith("multi_")):   # 2 x 3 K = 32
                 fl += -(-(nv * V // n) // 16) * 2 * 3 * 2.0 * 16 * 16 * 32
                 continue
-            kw = 32 if (2 * n > 16 and "k32" in kernel) else 16
+            kw = 32 if (2 * n > 16 and ("k32" in kernel or kernel.startswith("multi_"))) else 16
             tm, tk = (2 * n + 15) // 16, (2 * n + kw - 1) // kw
             fl += -(-(nv * V // n) // 16) * tm * tk * 3 * 2.0 * 16 * 16 * kw
         return fl

@@ -32,7 +32,7 @@ void register_mfma(std::vector<KernelDesc>& r)
     // race's tie-break), then the others.
     FX_MG4(6, 6, 6, 8, 256, 1, 0, 0, 1, "d");
     FX_MG4(8, 8, 8, 4, 256, 1, 0, 0, 1, "d");
-    FX_MG4(12, 12, 12, 2, 256, 1, 0, 4, 1, "p4d");
+    FX_MG4(12, 12, 12, 2, 256, 1, 32, 4, 1, "p4k32d");
     FX_MG4(17, 17, 17, 1, 256, 0, kFold17, 3, 0, "p3fold");
     FX_MG4(8, 8, 16, 4, 256, 1, 0, 0, 0, "");
     // One block per workgroup.
@@ -56,8 +56,10 @@ void register_mfma(std::vector<KernelDesc>& r)
     FX_MG4(6, 6, 6, 8, 256, 1, 0, 8, 1, "p8d");
     FX_MG4(8, 8, 8, 4, 256, 1, 0, 8, 1, "p8d");
     FX_MG4(12, 12, 12, 2, 256, 1, 0, 4, 0, "p4");
+    FX_MG4(12, 12, 12, 2, 256, 1, 0, 4, 1, "p4d");
     FX_MG4(17, 17, 17, 1, 256, 0, 0, 3, 0, "p3");
     FX_MG4(8, 8, 16, 4, 256, 1, 0, 8, 1, "p8d");
+    FX_MG4(8, 8, 16, 4, 256, 1, 32, 8, 1, "p8k32d");
 }
 
 }  // namespace fx

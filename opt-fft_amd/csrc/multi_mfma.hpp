@@ -27,15 +27,15 @@ struct MultiMfmaPreset {
     }
 };
 
-// One block of geometry G with its matrices built for it: 17^3 by the conjugate-pair fold;
-// the x pass stores straight to HBM for the short lines and through LDS for 17 (the faster
-// form of each).
+// One block of geometry G with its matrices built for it: 17^3 by the conjugate-pair fold,
+// the K = 32 instructions on the other axes with 2N > 16; the x pass stores straight to HBM
+// for the short lines and through LDS for 17 (the faster form of each).
 template <class G, int SPLIT>
 __device__ __forceinline__ void run_mfma_task(long bid, long nvol, const cx<float>* in,
                                               cx<float>* out, const Bank<float>* bk, float scl,
                                               bool conj, cx<float>* sm)
 {
-    constexpr int KW = (G::N0 == 17 && G::N1 == 17 && G::N2 == 17) ? kFold17 : 0;
+    constexpr int KW = (G::N0 == 17 && G::N1 == 17 && G::N2 == 17) ? kFold17 : 32;
     MfmaMats<G, SPLIT, KW> A;
     A.build(threadIdx.x & 63, bk);
     mfma_block<G, SPLIT, KW, (G::N0 > 12 ? 0 : 1)>(bid, nvol, in, out, scl, conj, A, sm);
