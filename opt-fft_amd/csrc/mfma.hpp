@@ -134,29 +134,36 @@ struct MfmaOps<1, 32> {
     }
 };
 
-// v and its value in lane l ^ 16 (gfx950 v_permlane16_swap: rows 0 <-> 1 and 2 <-> 3) or
-// l ^ 32 (v_permlane32_swap: rows 0, 1 <-> 2, 3); VALU moves, no LDS round trip.
-__device__ __forceinline__ unsigned xor16(unsigned v)
+// The lane exchanges below use gfx950's v_permlane16_swap (rows 0 <-> 1 and 2 <-> 3 of 16
+// lanes) and v_permlane32_swap (rows 0, 1 <-> 2, 3) with both operands holding v: every lane
+// gets back its own value and that of lane l ^ 16 (or l ^ 32), in an order that differs
+// between the rows.  VALU moves, no LDS round trip.  The two results are combined in integer
+// operations only: a float bit-cast of the second result compiled to a copy of the first in
+// this compiler's gfx950 code (r0 + r1 became 2 r0), while integer uses of both are kept.
+__device__ __forceinline__ unsigned max_xor16(unsigned v)
 {
     const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
-    return r[0] ^ r[1] ^ v;
+    return r[0] > r[1] ? r[0] : r[1];
 }
-__device__ __forceinline__ unsigned xor32(unsigned v)
+__device__ __forceinline__ unsigned max_xor32(unsigned v)
 {
     const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-    return r[0] ^ r[1] ^ v;
+    return r[0] > r[1] ? r[0] : r[1];
+}
+// v + v(l ^ 16), the partner's bits recovered as r0 ^ r1 ^ v; both lanes of a pair add the
+// same two values, in either order alike.
+__device__ __forceinline__ float sum_xor16(float v)
+{
+    const unsigned u = __builtin_bit_cast(unsigned, v);
+    const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    return v + __builtin_bit_cast(float, r[0] ^ r[1] ^ u);
 }
 
 // Largest of v over the four lanes that hold one line's operand values (l, l ^ 16, l ^ 32,
 // l ^ 48).  v >= 0, so its bit pattern orders like its value.
 __device__ __forceinline__ float line_max(float v)
 {
-    unsigned u = __builtin_bit_cast(unsigned, v);
-    unsigned w = xor16(u);
-    u = u > w ? u : w;
-    w = xor32(u);
-    u = u > w ? u : w;
-    return __builtin_bit_cast(float, u);
+    return __builtin_bit_cast(float, max_xor32(max_xor16(__builtin_bit_cast(unsigned, v))));
 }
 
 // Matrix fragments of one line length N for this lane: hi and lo parts, TM x TK tiles.
@@ -391,8 +398,8 @@ __device__ __forceinline__ void mfma_pass(const cx<float>* sm, int nlines, BaseF
             mx = fmaxf(mx, fmaxf(fabsf(v[2 * p]), fabsf(v[2 * p + 1])));
         }
         // X_0: the a sums of groups 0 and 1 (groups 2 and 3 exchange b sums, unused).
-        sr += __builtin_bit_cast(float, xor16(__builtin_bit_cast(unsigned, sr)));
-        si += __builtin_bit_cast(float, xor16(__builtin_bit_cast(unsigned, si)));
+        sr = sum_xor16(sr);
+        si = sum_xor16(si);
         float up = 1.f, down = scl;
         if constexpr (SPLIT == 1) {
             mx = line_max(mx);

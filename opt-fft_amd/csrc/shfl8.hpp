@@ -97,7 +97,9 @@ __device__ __forceinline__ unsigned dpp_move(unsigned v)
     return (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
 }
 // The value of lane l ^ M, with VALU moves (the permlane swaps return this lane's value and
-// its partner's in some order, so their XOR with this lane's value is the partner's).
+// its partner's in some order, so their XOR with this lane's value is the partner's; the
+// two results are combined in integer operations only: a float bit-cast of the second one
+// compiled to a copy of the first in this compiler's gfx950 code).
 template <int M>
 __device__ __forceinline__ unsigned lane_xor(unsigned v)
 {
