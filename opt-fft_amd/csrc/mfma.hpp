@@ -8,8 +8,7 @@
 // W[o][j] = exp(S 2 pi i o j / N), rows zero-padded to a multiple of 16 and columns to a
 // multiple of the instruction's K.  K is 16 (v_mfma_f32_16x16x16*) or 32 (gfx950
 // v_mfma_f32_16x16x32_*, KW = 32, for 2N > 16: a 17-point tile is 3 x 2 instructions instead
-// of 3 x 3, but its K pads 34 to 64 instead of 48, and those kernels hold more operand
-// registers; they stay in the race as the "k32" variants).  With the interleaved rows a
+// of 3 x 3, but its K pads 34 to 64 instead of 48; the race picks per geometry).  With the interleaved rows a
 // lane's operand values are whole complex points, one 8-byte LDS read each, at per-lane
 // offsets computed once per pass; the padded positions read point N-1 again (their matrix
 // columns are zero, so any finite value gives the same sums).  Both operands are split into
@@ -21,8 +20,10 @@
 // parts hold only |x| < 65504 and the low part of a small x falls below the normal range, so
 // with SPLIT = 1 every line is multiplied, before the split, by the power of two that brings
 // its largest magnitude into [0.5, 1) (a maximum over the four lanes holding the line,
-// through two lane-swap moves), and its outputs by the inverse power; both scalings are exact.  The matrix
-// fragments come from the plan-time bank and live in VGPRs for the whole pass.
+// through two lane-swap moves), and its outputs by the inverse power; both scalings are
+// exact.  The matrix fragments come from the plan-time bank, are built once per workgroup
+// and stay in VGPRs.  17-point lines may instead take the conjugate-pair fold (Fold17Matrix
+// below).
 //
 // Operand layouts, 64 lanes, Q = K/4 values per lane, q = 0..Q-1:
 //   A: lane l holds A[l % 16][Q (l / 16) + q];  B: B[Q (l / 16) + q][l % 16];
