@@ -45,8 +45,9 @@ void register_fused_17(std::vector<KernelDesc>& r)
     FX_17_BOTH(double, "c128", 1, 320, 1, "fold");
     FX_17_BOTH(double, "c128", 1, 320, 0, "rader");
     FX_17_BOTH(double, "c128", 1, 192, 1, "fold");
-    // (complex64 with the three-part prefetch, one volume P = 4 or two volumes P = 2: never
-    // picked over the first form below at 17^3 x 2048, 38.0 us; not registered.)
+    // (complex64 with the three-part prefetch, one volume P = 4 or two volumes P = 2, and
+    // three volumes per 960-thread workgroup, one per CU, with or without it (38.8 / 41.1
+    // us): never faster than the first form below at 17^3 x 2048, 38.0 us; not registered.)
     FX_17P_BOTH(float, "c64", 1, 320, 1, "fold", 4);
     FX_17S_BOTH(float, "c64", 1, 320, 1, "fold", 2, 1, 0, "_st");
     FX_17P_BOTH(float, "c64", 2, 640, 1, "fold", 2);
