@@ -42,8 +42,10 @@ __device__ __forceinline__ void run_mfma_task(long bid, long nvol, const cx<floa
 }
 
 // One task per workgroup (grid = ntasks; the hardware dispatcher hands tasks, heaviest
-// first, to whichever CU frees up).  A persistent loop over tasks would keep the matrix
-// fragments of several geometries live across iterations and spill.
+// first, to whichever CU frees up).  A persistent form drawing tasks from the claim queues
+// was built and measured: it spilled (130 VGPRs, lane-derived offsets of five geometries held
+// across the loop) until those were recomputed per task, and then took 205.7 us for the
+// bf16x3 mix against 190-196 us for this form; not kept.
 template <int SPLIT>
 __global__ void __launch_bounds__(MultiMfmaPreset::NT)
     __attribute__((amdgpu_waves_per_eu(MultiMfmaPreset::WPE, 8)))
