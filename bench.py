@@ -25,8 +25,9 @@ numpy.fft in float64, and for the matrix-core kernels the executed MFMA TFLOP/s 
 to back and host round trip, launched directly and replayed from a captured graph.
 
 "host_abi": the reference's host-pointer ABI (fft3d_create / fft3d_execute) at 17^3 x 1024:
-copies in and out of pageable host memory inside every execute, so the rate is the host
-link's, reported beside the device-resident value, never as it.
+copies in and out of pageable host memory inside every execute (through pinned staging, in
+pipelined chunks), so the rate is the host side's and the host link's, reported beside the
+device-resident value, never as it.
 
 "serving": the request server under Poisson arrivals (20,000 requests, the mixed stream of
 opt-fft_amd/serving.py, 64 GB budget, 16 streams, 30 us batching window for the
@@ -587,8 +588,8 @@ def host_abi(pkg, L=17, batch=1024, reps=8):
     """The reference's host-pointer ABI (fft3d_create / fft3d_execute, bench/geom/fft3d_api.h)
     on this library at the headline size: each execute copies the batch in from pageable host
     memory, transforms it and copies it back before returning, as bench/geom/driver.c
-    requires, so its rate is bounded by the host link, not HBM.  Median of `reps` executes
-    after one warm-up, output checked against numpy.fft."""
+    requires, so its rate is bounded by the host side and the host link, not HBM.  Median
+    of `reps` executes after one warm-up, output checked against numpy.fft."""
     f = pkg.fft3d
     L_ = f.load()
     x = make_input((L, L, L), batch, seed=9, dist="uniform")
@@ -609,7 +610,8 @@ def host_abi(pkg, L=17, batch=1024, reps=8):
     return {"L": L, "batch": batch, "ms_per_execute": t * 1e3, "transforms_per_s": batch / t,
             "host_link_GBs_each_way": x.nbytes / t / 1e9,
             "rel_l2_vs_numpy": _rel_l2(y, _np_dft(x, -1)),
-            "note": "pageable host buffers, H2D + transform + D2H per execute; not `value`"}
+            "note": "pageable host buffers: copies into pinned staging, H2D, transform, D2H "
+                    "and copies out per execute, pipelined in ~8 MB chunks; not `value`"}
 
 
 def main():

@@ -171,6 +171,29 @@ def test_host_pointer_abi(gpu):
     assert ctypes.c_char_p(L.fft3d_name()).value
 
 
+@pytest.mark.parametrize("n,batch", [(17, 300), (8, 4100), (36, 40)])
+def test_host_pointer_abi_over_several_chunks(gpu, n, batch):
+    """fft3d_execute runs the batch in ~8 MB chunks through pinned staging, two in flight:
+    batches of several chunks with a partial last one, against the oracle, twice on one
+    plan (the staging is reused), the input untouched."""
+    L = gpu.fft3d.load()
+    x = oracle.make_input((n, n, n), batch, seed=n + batch)
+    x0 = x.copy()
+    want = oracle.fftn_numpy(x)
+    p = L.fft3d_create(n, batch)
+    assert p
+    for rep in range(2):
+        out = np.full_like(x, np.nan)
+        L.fft3d_mx_clear_error()
+        L.fft3d_execute(p, x.ctypes.data, out.ctypes.data)
+        assert gpu.fft3d.last_error() == ""
+        err = oracle.rel_l2(out, want)
+        print(f"host ABI {n}^3 x {batch} rep {rep}: rel_l2={err:.2e}", flush=True)
+        assert err <= 1e-13, (n, batch, rep, err)
+    L.fft3d_destroy(p)
+    assert np.array_equal(x, x0)
+
+
 def test_device_pointer_abi(gpu):
     """The six fft3d_gpu_* symbols of bench/gpu/fft3d_gpu_api.h on device buffers."""
     L = gpu.fft3d.load()
