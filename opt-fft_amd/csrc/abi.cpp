@@ -167,7 +167,7 @@ extern "C" const char* fft3d_name(void) { return "mi355x_hip"; }
 extern "C" const char* fft3d_description(void)
 {
     return "MI355X gfx950 HIP: LDS-resident fused 3-axis kernels (L<=17, Rader-17), "
-           "LDS+register single pass or two-pass pencil (L=24), pencil (L=36); "
+           "residue-split single pass or two-pass pencil (L=24, 36); "
            "pipelined H2D+kernel+D2H per execute through pinned staging";
 }
 

@@ -2,8 +2,10 @@
 // 24x24x48) and their complex64 counterparts.  Names: s<SPW> slabs per workgroup,
 // t<NT> threads, b<NTB> threads of the x-line pass, c<CHUNK> volumes per launch pair
 // (0 = whole batch), p<P> persistent pipelined pass A with at most P workgroups per CU,
-// "_st" pass A with staggered HBM traffic (PencilCfg ST = 1).
+// "_st" pass A with staggered HBM traffic (PencilCfg ST = 1).  Also the residue-split
+// single-pass kernels of the same geometries (xres.hpp): "xres_..._v1r<R>t<NT>".
 #include "kernels.hpp"
+#include "xres.hpp"
 
 namespace fx {
 
@@ -31,6 +33,14 @@ namespace fx {
           "pencil_" #N0 "x" #N1 "x" #N2 "_" TAG "_fwd_s" #SPW "t" #NT "b" #NTB "c" #CH "p" #PIPE SUF "_r1"); \
     FX_PR(N0, N1, N2, T, +1, SPW, NT, PZ, NTB, CH, PIPE, ST,                                      \
           "pencil_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_s" #SPW "t" #NT "b" #NTB "c" #CH "p" #PIPE SUF "_r1")
+
+// Residue-split single pass (xres.hpp): "xres_<shape>_<prec>_<dir>_v1r<R>t<NT>", R workgroups
+// per volume, each holding N0 / R x-planes in LDS.
+#define FX_X(N0, N1, N2, T, S, R, NT, NAME) \
+    r.push_back(xres_desc<XResCfg<N0, N1, N2, T, S, R, NT>>(NAME))
+#define FX_X_BOTH(N0, N1, N2, T, TAG, R, NT)                                                    \
+    FX_X(N0, N1, N2, T, -1, R, NT, "xres_" #N0 "x" #N1 "x" #N2 "_" TAG "_fwd_v1r" #R "t" #NT); \
+    FX_X(N0, N1, N2, T, +1, R, NT, "xres_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_v1r" #R "t" #NT)
 
 // (Pass B with plain stores, PencilCfg XH = 1, measured 8% slower at 36^3 x 128 and 1% at
 // 24x24x48 x 64 than with nontemporal ones: not registered.)
@@ -64,6 +74,22 @@ void register_pencil(std::vector<KernelDesc>& r)
     FX_PR_BOTH(24, 24, 24, double, "c128", 8, 192, 1, 128, 0, 2, 0, "");
     FX_P_BOTH(24, 24, 48, float, "c64", 8, 384, 1, 128, 0, 2);
     FX_P_BOTH(24, 24, 48, float, "c64", 8, 256, 1, 128, 0, 0);
+    // Residue-split single pass (forced by name, rotated buffers, medians of >= 25 ms
+    // windows).  36^3 x 128: r6 (127.9 KB, one workgroup per CU) 64.6 us with 512 threads,
+    // 65.3 with 256, against 66.9 us for the best pencil; r12 (63.9 KB, two per CU, 12 reads
+    // of every element from L2) 84.7 us.  24x24x48 x 64: r4 (112.9 KB) 16.9 us with 384
+    // threads, 18.2 with 256, against 22.5 us; r6 / r8 (two per CU) 23.1 / 22.6 us.  24^3 x 512:
+    // r3 (76.8 KB, two per CU) 46.1 us with 256 threads, 48.5 with 192; r2 50.2, r4 55.8-56.1,
+    // against 53.3 us for the split-resident kernel; at batch 64 r4 / r3 10.4 / 10.5 us
+    // against 16.1 us for the pencil.  Thread counts keep every variant within its VGPR
+    // budget without scratch (a 48-point complex128 line takes ~200 VGPRs).
+    FX_X_BOTH(36, 36, 36, double, "c128", 6, 512);
+    FX_X_BOTH(36, 36, 36, double, "c128", 6, 256);
+    FX_X_BOTH(24, 24, 48, double, "c128", 4, 384);
+    FX_X_BOTH(24, 24, 48, double, "c128", 4, 256);
+    FX_X_BOTH(24, 24, 24, double, "c128", 3, 256);
+    FX_X_BOTH(24, 24, 24, double, "c128", 3, 192);
+    FX_X_BOTH(24, 24, 24, double, "c128", 4, 192);
 }
 
 }  // namespace fx

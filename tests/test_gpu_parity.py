@@ -119,6 +119,41 @@ def test_pipelined_17_variants_over_several_blocks(gpu):
         assert np.array_equal(y, y2), f"{name}: second execute differs"
 
 
+def test_residue_split_variants_over_several_volume_groups(gpu):
+    """Every residue-split variant, forced, at batch 19: workgroup g takes volume
+    8 * (g / 8 / R) + g % 8 and residue (g / 8) % R, so batch 19 runs two full groups of
+    eight volumes and a ragged third (three volumes, the other workgroups of that group
+    returning at once); at batch 1 and 5 only the first group exists.  Checked against the
+    oracle, and every output element is written (a NaN-filled output buffer must come
+    back finite)."""
+    f, h = gpu.fft3d, gpu.hip
+    names = [k for k in _kernels(gpu) if k[0].startswith("xres_")]
+    assert len(names) >= 6
+    batch = 19
+    ref = {}
+    for name, shape, prec, direction in names:
+        if (shape, direction) not in ref:
+            x = oracle.make_input(shape, batch, seed=19, dist="uniform", dtype=np.complex128)
+            ref[(shape, direction)] = (x, oracle.fftn_numpy(x, sign=direction))
+        x, want = ref[(shape, direction)]
+        print(f"{name} B={batch} ...", end=" ", flush=True)
+        plan = f.Plan(shape, batch, direction, prec, kernel=name)
+        assert plan.kernel == name
+        din, dout = h.DeviceBuffer(x.nbytes), h.DeviceBuffer(x.nbytes)
+        din.upload(x)
+        dout.upload(np.full_like(x, np.nan))
+        plan.execute(din.addr, dout.addr)
+        h.synchronize()
+        y = dout.download(x.dtype, x.shape)
+        din.close()
+        dout.close()
+        plan.close()
+        err = oracle.rel_l2(y, want)
+        print(f"rel_l2={err:.2e}", flush=True)
+        assert np.all(np.isfinite(y)), f"{name}: output element left unwritten"
+        assert err <= TOL[prec], f"{name}: rel_l2 {err:.3e}"
+
+
 def test_repeatable_and_input_untouched(gpu):
     f = gpu.fft3d
     h = gpu.hip
