@@ -82,7 +82,11 @@ void register_pencil(std::vector<KernelDesc>& r)
     // r3 (76.8 KB, two per CU) 46.1 us with 256 threads, 48.5 with 192; r2 50.2, r4 55.8-56.1,
     // against 53.3 us for the split-resident kernel; at batch 64 r4 / r3 10.4 / 10.5 us
     // against 16.1 us for the pencil.  Thread counts keep every variant within its VGPR
-    // budget without scratch (a 48-point complex128 line takes ~200 VGPRs).
+    // budget without scratch (a 48-point complex128 line takes ~200 VGPRs).  At 36^3 a
+    // workgroup's x pass (reading 746 KB from L2 at ~52 GB/s per CU) takes 14.4 us of its
+    // ~20 us; a persistent form computing the next task's x pass into registers during the
+    // LDS passes (256 threads, 486 VGPRs) ran at 165 us: at that register pressure its loads
+    // were issued one or two at a time, and issuing them together spilled.  Not kept.
     FX_X_BOTH(36, 36, 36, double, "c128", 6, 512);
     FX_X_BOTH(36, 36, 36, double, "c128", 6, 256);
     FX_X_BOTH(24, 24, 48, double, "c128", 4, 384);
