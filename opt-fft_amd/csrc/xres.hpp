@@ -1,5 +1,6 @@
 // Single-pass kernel for volumes larger than a CU holds (36^3 complex128, 746 KB; 24x24x48
-// complex128, 442 KB): the x DFT split by output residue.  With N0 = R * M, output x-plane
+// complex128, 442 KB; also 24^3 complex128 and the complex64 36^3 and 24x24x48): the x DFT
+// split by output residue.  With N0 = R * M, output x-plane
 // R q + h of X[k] = sum_j x[j] W^(j k) is
 //
 //   X[R q + h] = sum_{n < M} W_M^(n q) * ( W_N0^(n h) * sum_{m < R} x[n + M m] W_R^(m h) ),

@@ -70,6 +70,7 @@ def test_long_double_oracle_agrees_at_small_sizes(gpu):
     ((17, 17, 17), 1024, "uniform"),
     ((36, 36, 36), 128, "normal_half"),
     ((24, 24, 48), 64, "normal_half"),
+    ((24, 24, 24), 512, "uniform"),
 ])
 def test_workload_sizes_forward_and_inverse(gpu, shape, batch, dist):
     """Full workload sizes: forward vs numpy.fft, and backward(normalised) round trip."""
@@ -132,10 +133,11 @@ def test_residue_split_variants_over_several_volume_groups(gpu):
     batch = 19
     ref = {}
     for name, shape, prec, direction in names:
-        if (shape, direction) not in ref:
-            x = oracle.make_input(shape, batch, seed=19, dist="uniform", dtype=np.complex128)
-            ref[(shape, direction)] = (x, oracle.fftn_numpy(x, sign=direction))
-        x, want = ref[(shape, direction)]
+        if (shape, prec, direction) not in ref:
+            dt = np.complex128 if prec == f.C128 else np.complex64
+            x = oracle.make_input(shape, batch, seed=19, dist="uniform", dtype=dt)
+            ref[(shape, prec, direction)] = (x, oracle.fftn_numpy(x, sign=direction))
+        x, want = ref[(shape, prec, direction)]
         print(f"{name} B={batch} ...", end=" ", flush=True)
         plan = f.Plan(shape, batch, direction, prec, kernel=name)
         assert plan.kernel == name
