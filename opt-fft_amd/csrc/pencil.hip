@@ -36,11 +36,17 @@ namespace fx {
 
 // Residue-split single pass (xres.hpp): "xres_<shape>_<prec>_<dir>_v1r<R>t<NT>", R workgroups
 // per volume, each holding N0 / R x-planes in LDS.
-#define FX_X(N0, N1, N2, T, S, R, NT, NAME) \
-    r.push_back(xres_desc<XResCfg<N0, N1, N2, T, S, R, NT>>(NAME))
-#define FX_X_BOTH(N0, N1, N2, T, TAG, R, NT)                                                    \
-    FX_X(N0, N1, N2, T, -1, R, NT, "xres_" #N0 "x" #N1 "x" #N2 "_" TAG "_fwd_v1r" #R "t" #NT); \
-    FX_X(N0, N1, N2, T, +1, R, NT, "xres_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_v1r" #R "t" #NT)
+#define FX_X(N0, N1, N2, T, S, R, NT, OR, NAME) \
+    r.push_back(xres_desc<XResCfg<N0, N1, N2, T, S, R, NT, OR>>(NAME))
+#define FX_X_BOTH(N0, N1, N2, T, TAG, R, NT)                                                       \
+    FX_X(N0, N1, N2, T, -1, R, NT, 0, "xres_" #N0 "x" #N1 "x" #N2 "_" TAG "_fwd_v1r" #R "t" #NT); \
+    FX_X(N0, N1, N2, T, +1, R, NT, 0, "xres_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_v1r" #R "t" #NT)
+// "o<OR>": the x pass's point order (XResCfg OR: 1 rotated thread blocks, 2 offset starts).
+#define FX_XO_BOTH(N0, N1, N2, T, TAG, R, NT, OR)                                                \
+    FX_X(N0, N1, N2, T, -1, R, NT, OR,                                                          \
+         "xres_" #N0 "x" #N1 "x" #N2 "_" TAG "_fwd_v1r" #R "t" #NT "o" #OR);                   \
+    FX_X(N0, N1, N2, T, +1, R, NT, OR,                                                          \
+         "xres_" #N0 "x" #N1 "x" #N2 "_" TAG "_bwd_v1r" #R "t" #NT "o" #OR)
 
 // (Pass B with plain stores, PencilCfg XH = 1, measured 8% slower at 36^3 x 128 and 1% at
 // 24x24x48 x 64 than with nontemporal ones: not registered.)
@@ -82,15 +88,23 @@ void register_pencil(std::vector<KernelDesc>& r)
     // r3 (76.8 KB, two per CU) 46.1 us with 256 threads, 48.5 with 192; r2 50.2, r4 55.8-56.1,
     // against 53.3 us for the split-resident kernel; at batch 64 r4 / r3 10.4 / 10.5 us
     // against 16.1 us for the pencil.  Thread counts keep every variant within its VGPR
-    // budget without scratch (a 48-point complex128 line takes ~200 VGPRs).  At 36^3 a
+    // budget without scratch (a 48-point complex128 line takes ~200 VGPRs).  Point order
+    // of the x pass (one device): 36^3 x 128 57.1 us with offset starts (o2), 57.5 rotated
+    // blocks (o1), 65.2 from point 0; 24x24x48 x 64 16.6 (o2) against 17.6 us; 24^3 x 512
+    // 48.0 (o1) against 49.8 us; 36^3 complex64 x 64 19.8 (o2) against 20.6 us; 24x24x48
+    // complex64 x 256 30.0 (o2) against 29.1 us, not registered.  At 36^3 a
     // workgroup's x pass (reading 746 KB from L2 at ~52 GB/s per CU) takes 14.4 us of its
     // ~20 us; a persistent form computing the next task's x pass into registers during the
     // LDS passes (256 threads, 486 VGPRs) ran at 165 us: at that register pressure its loads
     // were issued one or two at a time, and issuing them together spilled.  Not kept.
+    FX_XO_BOTH(36, 36, 36, double, "c128", 6, 512, 2);
+    FX_XO_BOTH(36, 36, 36, double, "c128", 6, 512, 1);
     FX_X_BOTH(36, 36, 36, double, "c128", 6, 512);
     FX_X_BOTH(36, 36, 36, double, "c128", 6, 256);
+    FX_XO_BOTH(24, 24, 48, double, "c128", 4, 384, 2);
     FX_X_BOTH(24, 24, 48, double, "c128", 4, 384);
     FX_X_BOTH(24, 24, 48, double, "c128", 4, 256);
+    FX_XO_BOTH(24, 24, 24, double, "c128", 3, 256, 1);
     FX_X_BOTH(24, 24, 24, double, "c128", 3, 256);
     FX_X_BOTH(24, 24, 24, double, "c128", 3, 192);
     FX_X_BOTH(24, 24, 24, double, "c128", 4, 192);
@@ -100,6 +114,7 @@ void register_pencil(std::vector<KernelDesc>& r)
     // 21.6 us for the pencil.
     FX_X_BOTH(24, 24, 48, float, "c64", 2, 384);
     FX_X_BOTH(24, 24, 48, float, "c64", 3, 256);
+    FX_XO_BOTH(36, 36, 36, float, "c64", 3, 512, 2);
     FX_X_BOTH(36, 36, 36, float, "c64", 3, 512);
 }
 

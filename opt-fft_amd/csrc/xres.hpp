@@ -20,11 +20,19 @@
 
 namespace fx {
 
-template <int N0_, int N1_, int N2_, typename T_, int S_, int R_, int NT_, int PZ_ = 1,
-          int HL_ = 0, int HS_ = 1, int ALG_ = 0>
+// OR: order of the x pass's points.  0: every workgroup of a volume walks the plane from
+// point 0, so the R of them request the same lines together; 1: residue h starts at its
+// thread block h mod PPT (the blocks of NT points taken in rotated order); 2: residue h
+// starts at point h * PLANE / R (the plane walked with a wrap).  1 and 2 spread the R
+// workgroups' requests over different lines (and L2 channels) at the price of keeping each
+// line in L2 longer.
+template <int N0_, int N1_, int N2_, typename T_, int S_, int R_, int NT_, int OR_ = 0,
+          int PZ_ = 1, int HL_ = 0, int HS_ = 1, int ALG_ = 0>
 struct XResCfg {
     using T = T_;
     static constexpr int N0 = N0_, N1 = N1_, N2 = N2_, S = S_, R = R_, NT = NT_, PZ = PZ_;
+    static constexpr int OR = OR_;
+    static constexpr int PPT = (N1 * N2 + NT - 1) / NT;   // (y, z) points per thread
     static constexpr int HL = HL_, HS = HS_, ALG = ALG_, V17 = ALG_, VPW = 1, P = 0;
     static_assert(N0 % R == 0, "the residues split the x-planes evenly");
     static constexpr int M = N0 / R;       // x-planes per workgroup
@@ -70,7 +78,19 @@ struct XRes {
         for (int m = 1; m < C::R; ++m) wr[m] = w0(C::M * ((m * h) % C::R), bk);
 #pragma unroll
         for (int n = 1; n < C::M; ++n) tw[n] = w0(n * h, bk);
-        for (int p = tid; p < C::PLANE; p += C::NT) {
+        for (int kk = 0; kk < C::PPT; ++kk) {
+            int p;
+            if constexpr (C::OR == 1) {
+                p = tid + ((kk + h) % C::PPT) * C::NT;
+                if (p >= C::PLANE) continue;
+            } else {
+                p = tid + kk * C::NT;
+                if (p >= C::PLANE) continue;
+                if constexpr (C::OR == 2) {
+                    p += h * (C::PLANE / C::R);
+                    p = p < C::PLANE ? p : p - C::PLANE;
+                }
+            }
             cx<T> a[C::M];
 #pragma unroll
             for (int n = 0; n < C::M; ++n) a[n] = ld_hint<C::HL>(src + n * C::PLANE + p);
