@@ -1,7 +1,7 @@
 // Single-pass kernel for volumes larger than a CU holds (36^3 complex128, 746 KB; 24x24x48
 // complex128, 442 KB; also 24^3 complex128 and the complex64 36^3 and 24x24x48): the x DFT
-// split by output residue.  With N0 = R * M, output x-plane
-// R q + h of X[k] = sum_j x[j] W^(j k) is
+// split by output residue.  With N0 = R * M, output x-plane R q + h of
+// X[k] = sum_j x[j] W^(j k) is
 //
 //   X[R q + h] = sum_{n < M} W_M^(n q) * ( W_N0^(n h) * sum_{m < R} x[n + M m] W_R^(m h) ),
 //
@@ -11,9 +11,10 @@
 // M planes (N1 x N2, rows padded by PZ) are held in LDS, z lines and then y lines are
 // transformed there, and the y pass writes planes R q + h to HBM.  Every element is written
 // once; it is read R times, by the R workgroups of its volume, which are consecutive
-// workgroups of one XCD (an 8-XCD part dispatches workgroup g to XCD g mod 8), so they start
-// together and all but the first read of a line are served by that XCD's L2.  The inputs
-// are therefore read with plain loads (HL = 0), the outputs stored nontemporal (HS = 1).
+// workgroups of one XCD (an 8-XCD part dispatches workgroup g to XCD g mod 8), so they run
+// together and most reads of a line after the first are served by that XCD's L2.  The
+// inputs are therefore read with plain loads (HL = 0), the outputs stored nontemporal
+// (HS = 1).  What bounds the kernel is the rate at which one CU reads from L2.
 #pragma once
 
 #include "kernels.hpp"
