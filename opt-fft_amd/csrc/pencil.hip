@@ -96,7 +96,10 @@ void register_pencil(std::vector<KernelDesc>& r)
     // workgroup's x pass (reading 746 KB from L2 at ~52 GB/s per CU) takes 14.4 us of its
     // ~20 us; a persistent form computing the next task's x pass into registers during the
     // LDS passes (256 threads, 486 VGPRs) ran at 165 us: at that register pressure its loads
-    // were issued one or two at a time, and issuing them together spilled.  Not kept.
+    // were issued one or two at a time, and issuing them together spilled.  A second form
+    // (512 threads, 169 VGPRs: two-stage 4 x 9 line passes, loads six at a time, offset
+    // starts) ran without scratch but took 69.8 us at x 128 and 251.4 at x 512 against 57.0 /
+    // 225.3 us for the one-task form.  Not kept.
     FX_XO_BOTH(36, 36, 36, double, "c128", 6, 512, 2);
     FX_XO_BOTH(36, 36, 36, double, "c128", 6, 512, 1);
     FX_X_BOTH(36, 36, 36, double, "c128", 6, 512);
